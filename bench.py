@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X DPD engine on the polymer world (32x32x32 cells,
+density 3, 98,304 beads, 10% of beads in H2T4 chains), one GPU.
+
+bead-steps/s = beads x steps / wall time of the timed run, state resident in
+HBM (the reference's benchmark_engine.cpp:128-147 rule with Attach excluded).
+Prints ONE JSON line. Extra keys:
+  roofline      pair-force kernel: algorithmic bytes / its average duration
+                (HIP events on the engine's stream, profiled pass of the same run)
+  cpu_baseline  the reference's NaiveDPDEngineHalfMerge compiled from its
+                sources (-O3 -ffast-math), one thread, bounded sample
+  cpu_port_mt   the repository's multi-threaded half-merge restatement
+  batched       (--batch) mixed-size ensemble, one launch set per step vs the
+                same worlds one at a time
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "poets-dpd_amd"))
+
+METRIC = "bead-steps/s on 1 MI355X, batched vs plain engine; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD public specification
+FP32_PEAK_TFLOPS = 157.3
+FORCE_BYTES_PER_BEAD = 40      # read x, v, key (28 B) + write f (12 B)
+STEP_BYTES_PER_BEAD = 80       # algorithmic bytes of a whole step per bead
+FLOP_PER_BEAD_STEP = 282       # 6.283 pairs x 41 + 24
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per pair-force launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", "pair_force_polymer32_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(world, target_s):
+    import oracle as O   # the checker's tree, used here only to time the CPU baseline
+    out = {}
+    if O.ref_available(fast=True):
+        timing = {}
+        steps = 1
+        O.ref_run(world, "half_merge", steps, fast=True, timing=timing)
+        per = max(timing["run_s"], 1e-6)
+        steps = max(1, min(2000, int(target_s / per)))
+        O.ref_run(world, "half_merge", steps, fast=True, timing=timing)
+        out["cpu_baseline"] = {
+            "value": world.n_beads * steps / timing["run_s"], "unit": "bead-steps/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"NaiveDPDEngineHalfMerge built from the reference sources (-O3 -ffast-math), "
+                      f"same polymer world, {steps} steps, Run timed alone"}
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    st = O.F64State(world)
+    t0 = time.perf_counter()
+    st.run_half_merge(1, nthreads=nthreads)
+    per = max(time.perf_counter() - t0, 1e-6)
+    steps = max(1, min(2000, int(target_s / per)))
+    st = O.F64State(world)
+    t0 = time.perf_counter()
+    st.run_half_merge(steps, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    port = {"value": world.n_beads * steps / dt, "unit": "bead-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"multi-threaded FP64 half-merge restatement, same world, {steps} steps"}
+    if "cpu_baseline" not in out:
+        out["cpu_baseline"] = port
+    else:
+        out["cpu_port_mt"] = port
+    return out
+
+
+def run_batched(args, ctx_cls, W):
+    """Mixed-size ensemble: all worlds advance in one launch set per step, vs the
+    same worlds one at a time through the plain engine."""
+    worlds = W.config_batch(n_worlds=args.batch, seed=1)
+    total = sum(w.n_beads for w in worlds)
+    steps = args.batch_steps
+    ctx = ctx_cls(0)
+    for w in worlds:
+        ctx.add(w)
+    tune = ctx.autotune(10) if args.autotune else "heuristic"
+    ctx.run(2)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.run(steps)
+    ctx.sync()
+    tb = time.perf_counter() - t0
+    batched = total * steps / tb
+    ctx.close()
+    # one at a time, a sample of the worlds, same steps
+    k = min(len(worlds), args.batch_solo_sample)
+    solo_time = 0.0
+    solo_beads = 0
+    for w in worlds[:k]:
+        c = ctx_cls(0)
+        c.add(w)
+        c.run(2)
+        c.sync()
+        t0 = time.perf_counter()
+        c.run(steps)
+        c.sync()
+        solo_time += time.perf_counter() - t0
+        solo_beads += w.n_beads
+        c.close()
+    solo = solo_beads * steps / solo_time
+    return {"worlds": len(worlds), "beads": total, "steps": steps, "value": batched, "unit": "bead-steps/s",
+            "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "tuning": tune}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, help="also run the batched ensemble of this many worlds")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-solo-sample", type=int, default=32)
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--side", type=int, default=32)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent worlds shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pdpd_amd import capi
+    from pdpd_amd import world as W
+    capi.load()
+
+    world = W.config_polymer(seed=1, L=args.side)
+    n = world.n_beads
+    log(f"bench: polymer world {args.side}^3, {n} beads; warmup {args.warmup}, timed {args.steps}")
+    ctx = capi.Context(0)
+    slot = ctx.add(world)
+    if args.autotune:
+        log("bench: autotune " + ctx.autotune(20))
+    ctx.run(args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.run(args.steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    log(f"bench: {value:.4e} bead-steps/s ({elapsed * 1e3 / args.steps:.4f} ms/step)")
+
+    ctx.run_profiled(args.profile_steps)
+    st = ctx.stats()
+    force_ms = st["k_force_ms"]
+    step_ms = sum(st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms", "k_reorder_ms",
+                                  "k_force_ms", "k_bonds_ms"))
+    achieved = n * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "k_force", "kernel_ms": force_ms,
+                "valu_fp32_frac": n * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
+                "step_hbm_frac": n * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                "kernel_ms_per_step": {k[2:-3]: st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms",
+                                                               "k_reorder_ms", "k_force_ms", "k_bonds_ms")},
+                "brick": list(st["brick"])}
+    x, v, f, t = ctx.download(slot)
+    ctx.close()
+
+    out = {"metric": METRIC, "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"plain engine, polymer world {args.side}x{args.side}x{args.side} cells, "
+                                  f"density 3, {n} beads, 10% in H2T4 chains (Hookean k=128 r0=0.5, angle k=20)",
+                      "beads": n, "dt": 0.01, "engine": "HipDPDEngine (graph-captured steps)"},
+           "roofline": roofline}
+    if args.batch:
+        log(f"bench: batched ensemble of {args.batch} worlds")
+        out["batched"] = run_batched(args, capi.Context, W)
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline")
+        out.update(cpu_baseline(world, args.cpu_seconds))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,427 @@
+// hip_dpd_engine.hpp -- DPDEngine subclasses that run on an MI355X through the
+// C ABI of libpdpd_hip.so (include/pdpd_hip.h).
+//
+//   HipDPDEngine        plain engine: one WorldState, the reference's
+//                       CanSupport / Attach / Run(n) / Run(intervals) surface
+//                       (include/dpd/core/dpd_engine.hpp:57-215 in poets-dpd)
+//   HipBatchDPDEngine   many WorldStates of uneven size advanced together, one
+//                       launch set per step (the batched pattern; no counterpart
+//                       in the reference)
+//
+// The header compiles against this repository's restatement of the engine
+// interface (pdpd/dpd_engine.hpp) or, with PDPD_HOST_REFERENCE defined and the
+// reference's include/ on the path, against poets-dpd's own
+// dpd/core/dpd_engine.hpp: it touches only the WorldState fields both share.
+#pragma once
+
+#if defined(PDPD_HOST_REFERENCE)
+#include "dpd/core/dpd_engine.hpp"
+#else
+#include "pdpd/dpd_engine.hpp"
+#endif
+#include "pdpd_hip.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace pdpd_host {
+
+inline void check(int rc, const char *what)
+{
+    if (rc != PDPD_OK) throw std::runtime_error(std::string(what) + ": " + pdpd_last_error());
+}
+
+// WorldState -> pdpd_world_desc. Owns the flattened arrays the descriptor
+// points into; valid while this object lives.
+struct WorldDescBuffers {
+    pdpd_world_desc desc{};
+    std::vector<double> cons, diss, x, v, f, bond_kappa, bond_r0, angle_kappa, angle_theta0;
+    std::vector<uint8_t> stationary, bead_type, is_monomer, polymer_offset;
+    std::vector<uint32_t> polymer_id, pt_bond_start, bond_head, bond_tail, pt_angle_start, angle_head_bond,
+        angle_tail_bond, polymer_type, polymer_bead_start, polymer_bead_ids;
+
+    explicit WorldDescBuffers(const WorldState &s)
+    {
+        const size_t nt = s.bead_types.size();
+        for (size_t i = 0; i < s.interactions.size(); i++) {
+            cons.push_back((double)s.interactions[i].conservative);
+            diss.push_back((double)s.interactions[i].dissipative);
+        }
+        bool any_stationary = false;
+        for (const auto &bt : s.bead_types) {
+            stationary.push_back(bt.stationary ? 1 : 0);
+            any_stationary |= bt.stationary;
+        }
+        const size_t n = s.beads.size();
+        x.resize(3 * n); v.resize(3 * n); f.resize(3 * n);
+        bead_type.resize(n); is_monomer.resize(n); polymer_offset.resize(n); polymer_id.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            const auto &b = s.beads[i];
+            for (int d = 0; d < 3; d++) { x[3 * i + d] = b.x[d]; v[3 * i + d] = b.v[d]; f[3 * i + d] = b.f[d]; }
+            bead_type[i] = b.bead_type;
+            is_monomer[i] = b.is_monomer ? 1 : 0;
+            polymer_offset[i] = b.polymer_offset;
+            polymer_id[i] = b.polymer_id;
+        }
+        pt_bond_start.push_back(0);
+        pt_angle_start.push_back(0);
+        for (const auto &pt : s.polymer_types) {
+            for (const auto &b : pt.bonds) {
+                bond_head.push_back(b.bead_offset_head);
+                bond_tail.push_back(b.bead_offset_tail);
+                bond_kappa.push_back((double)b.kappa);
+                bond_r0.push_back((double)b.r0);
+            }
+            pt_bond_start.push_back((uint32_t)bond_head.size());
+            for (const auto &bp : pt.bond_pairs) {
+                angle_head_bond.push_back(bp.bond_offset_head);
+                angle_tail_bond.push_back(bp.bond_offset_tail);
+                angle_kappa.push_back((double)bp.kappa);
+                angle_theta0.push_back((double)bp.theta0);
+            }
+            pt_angle_start.push_back((uint32_t)angle_head_bond.size());
+        }
+        polymer_bead_start.push_back(0);
+        for (const auto &p : s.polymers) {
+            polymer_type.push_back(p.polymer_type);
+            for (auto id : p.bead_ids) polymer_bead_ids.push_back(id);
+            polymer_bead_start.push_back((uint32_t)polymer_bead_ids.size());
+        }
+        pdpd_world_desc &d = desc;
+        for (int k = 0; k < 3; k++) d.box[k] = s.box[k];
+        d.dt = s.dt;
+        d.lambda = s.lambda;
+        d.seed = s.seed;
+        d.t = (int64_t)s.t;
+        d.n_bead_types = (uint32_t)nt;
+        d.conservative = cons.data();
+        d.dissipative = diss.data();
+        d.stationary = any_stationary ? stationary.data() : nullptr;
+        d.n_beads = (uint32_t)n;
+        d.x = x.data(); d.v = v.data(); d.f = f.data();
+        d.bead_type = bead_type.data();
+        d.is_monomer = is_monomer.data();
+        d.polymer_id = polymer_id.data();
+        d.polymer_offset = polymer_offset.data();
+        d.n_polymer_types = (uint32_t)s.polymer_types.size();
+        d.pt_bond_start = pt_bond_start.data();
+        d.bond_head = bond_head.data(); d.bond_tail = bond_tail.data();
+        d.bond_kappa = bond_kappa.data(); d.bond_r0 = bond_r0.data();
+        d.pt_angle_start = pt_angle_start.data();
+        d.angle_head_bond = angle_head_bond.data(); d.angle_tail_bond = angle_tail_bond.data();
+        d.angle_kappa = angle_kappa.data(); d.angle_theta0 = angle_theta0.data();
+        d.n_polymers = (uint32_t)s.polymers.size();
+        d.polymer_type = polymer_type.data();
+        d.polymer_bead_start = polymer_bead_start.data();
+        d.polymer_bead_ids = polymer_bead_ids.data();
+    }
+};
+
+// Host copy of what the engine last wrote into a WorldState, to notice edits
+// made between runs (a caller may change velocities without re-attaching).
+struct Snapshot {
+    std::vector<double> xvf;
+    long t = 0;
+    void take(const WorldState &s)
+    {
+        xvf.resize(9 * s.beads.size());
+        for (size_t i = 0; i < s.beads.size(); i++)
+            for (int d = 0; d < 3; d++) {
+                xvf[9 * i + d] = s.beads[i].x[d];
+                xvf[9 * i + 3 + d] = s.beads[i].v[d];
+                xvf[9 * i + 6 + d] = s.beads[i].f[d];
+            }
+        t = s.t;
+    }
+    bool same(const WorldState &s) const
+    {
+        if (xvf.size() != 9 * s.beads.size() || t != s.t) return false;
+        for (size_t i = 0; i < s.beads.size(); i++)
+            for (int d = 0; d < 3; d++)
+                if (xvf[9 * i + d] != s.beads[i].x[d] || xvf[9 * i + 3 + d] != s.beads[i].v[d] ||
+                    xvf[9 * i + 6 + d] != s.beads[i].f[d])
+                    return false;
+        return true;
+    }
+};
+
+// Device context shared by the engines (one per engine instance).
+class Context {
+public:
+    explicit Context(int device) : m_device(device) {}
+    ~Context() { if (m_ctx) pdpd_ctx_destroy(m_ctx); }
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+
+    pdpd_ctx *get()
+    {
+        if (!m_ctx) check(pdpd_ctx_create(m_device, &m_ctx), "pdpd_ctx_create");
+        return m_ctx;
+    }
+    bool created() const { return m_ctx != nullptr; }
+
+private:
+    int m_device;
+    pdpd_ctx *m_ctx = nullptr;
+};
+
+inline void download_into(pdpd_ctx *ctx, int slot, WorldState &s)
+{
+    const size_t n = s.beads.size();
+    std::vector<double> x(3 * n), v(3 * n), f(3 * n);
+    int64_t t = 0;
+    check(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t), "pdpd_world_download");
+    for (size_t i = 0; i < n; i++) {
+        auto &b = s.beads[i];
+        for (int d = 0; d < 3; d++) { b.x[d] = x[3 * i + d]; b.v[d] = v[3 * i + d]; b.f[d] = f[3 * i + d]; }
+    }
+    s.t = (long)t;
+}
+
+}  // namespace pdpd_host
+
+class HipDPDEngine : public DPDEngine {
+protected:
+    bool CanSupportHookeanBonds() const override { return true; }
+    bool CanSupportAngleBonds() const override { return true; }
+
+public:
+    explicit HipDPDEngine(int device = 0, unsigned graph_steps = 20) : m_ctx(device), m_graph_steps(graph_steps) {}
+
+    ~HipDPDEngine() override
+    {
+        if (m_ctx.created() && m_slot >= 0) pdpd_world_remove(m_ctx.get(), m_slot);
+    }
+
+    std::string CanSupport(const WorldState *s) const override
+    {
+        std::string r = DPDEngine::CanSupport(s);
+        if (!r.empty()) return r;
+        pdpd_host::WorldDescBuffers b(*s);
+        return pdpd_can_support(&b.desc);
+    }
+
+    // Bonds use the minimum image, so any bond shorter than half the box works.
+    double GetMaxBondLength() const override { return 1000; }
+
+    void PrepareResources() override { m_ctx.get(); }
+
+    bool GetTimings(timings_t &t) override
+    {
+        if (!m_ctx.created()) return false;
+        pdpd_stats st;
+        pdpd_host::check(pdpd_get_stats(m_ctx.get(), &st), "pdpd_get_stats");
+        t = timings_t{};
+        t.configure = st.upload_s;
+        t.execute_to_first_bead = st.run_s;
+        t.execute_to_last_bead = st.run_s;
+        return true;
+    }
+
+    void GetConfigEntries(std::vector<engine_config_entry_t> &config) override
+    {
+        config.clear();
+        std::vector<char> buf(4096);
+        pdpd_host::check(pdpd_list_config(ctx(), buf.data(), buf.size()), "pdpd_list_config");
+        std::string all(buf.data());
+        size_t pos = 0;
+        while (pos < all.size()) {
+            size_t eol = all.find('\n', pos);
+            std::string line = all.substr(pos, eol == std::string::npos ? std::string::npos : eol - pos);
+            pos = eol == std::string::npos ? all.size() : eol + 1;
+            size_t eq = line.find('='), tab = line.find('\t');
+            if (eq == std::string::npos || tab == std::string::npos) continue;
+            config.push_back({line.substr(0, eq), line.substr(tab + 1), line.substr(eq + 1, tab - eq - 1)});
+        }
+    }
+
+    void SetConfigEntry(const std::string &key, const std::string &value) override
+    {
+        if (pdpd_set_config(ctx(), key.c_str(), value.c_str()) != PDPD_OK)
+            throw std::runtime_error(pdpd_last_error());
+    }
+
+    void Attach(WorldState *state) override
+    {
+        if (m_slot >= 0) {
+            pdpd_host::check(pdpd_world_remove(ctx(), m_slot), "pdpd_world_remove");
+            m_slot = -1;
+        }
+        m_state = state;
+        if (!state) return;
+        std::string why = CanSupport(state);
+        if (!why.empty()) throw std::runtime_error("Can't support this world : " + why);
+        import_state();
+    }
+
+    void Run(unsigned nSteps) override
+    {
+        require_attached();
+        if (!m_snap.same(*m_state)) reimport();
+        pdpd_host::check(pdpd_run(ctx(), nSteps), "pdpd_run");
+        export_state();
+    }
+
+    // Interval run with the next interval already on the device while the
+    // callback of the previous one runs on the host (the contract of
+    // dpd_engine.hpp:187-194 allows execution to continue during the callback).
+    unsigned Run(int interval_count, unsigned interval_size, std::function<bool()> interval_callback) override
+    {
+        require_attached();
+        if (interval_count == 0) return 0;
+        if (!m_snap.same(*m_state)) reimport();
+        unsigned done = 0;
+        pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
+        for (;;) {
+            export_state();
+            done += interval_size;
+            const bool more = interval_count < 0 || interval_count > 1;
+            if (more) pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
+            const bool cont = interval_callback();
+            if (interval_count > 0) --interval_count;
+            if (more && !m_snap.same(*m_state)) {
+                // the callback edited the world: drop the speculative interval
+                pdpd_host::check(pdpd_sync(ctx()), "pdpd_sync");
+                reimport();
+                if (cont) pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
+            }
+            if (!cont || !more) {
+                if (more && cont == false) m_device_ahead = true;
+                break;
+            }
+        }
+        if (m_device_ahead) {
+            pdpd_host::check(pdpd_sync(ctx()), "pdpd_sync");
+            reimport();
+            m_device_ahead = false;
+        }
+        return done;
+    }
+
+protected:
+    pdpd_ctx *ctx() { return m_ctx.get(); }
+
+    void require_attached() const
+    {
+        if (!m_state || m_slot < 0) throw std::runtime_error("HipDPDEngine: no world attached");
+    }
+
+    void import_state()
+    {
+        pdpd_host::WorldDescBuffers b(*m_state);
+        pdpd_ctx *c = ctx();
+        pdpd_host::check(pdpd_set_config(c, "graph_steps", std::to_string(m_graph_steps).c_str()), "pdpd_set_config");
+        pdpd_host::check(pdpd_world_add(c, &b.desc, &m_slot), "pdpd_world_add");
+        m_snap.take(*m_state);
+    }
+
+    void reimport()
+    {
+        if (m_slot >= 0) pdpd_host::check(pdpd_world_remove(ctx(), m_slot), "pdpd_world_remove");
+        m_slot = -1;
+        import_state();
+    }
+
+    void export_state()
+    {
+        pdpd_host::download_into(ctx(), m_slot, *m_state);
+        m_snap.take(*m_state);
+    }
+
+    pdpd_host::Context m_ctx;
+    unsigned m_graph_steps;
+    WorldState *m_state = nullptr;
+    int m_slot = -1;
+    pdpd_host::Snapshot m_snap;
+    bool m_device_ahead = false;
+};
+
+// Many independent worlds advanced together: every step is one launch set over
+// all of them. Worlds may be attached and detached between runs; each keeps its
+// own t, seed and parameters, and its results equal a solo run bit for bit.
+class HipBatchDPDEngine {
+public:
+    explicit HipBatchDPDEngine(int device = 0) : m_ctx(device) {}
+
+    std::string CanSupport(const WorldState *s) const
+    {
+        HipDPDEngine probe;
+        return probe.CanSupport(s);
+    }
+
+    // Attach one more world; returns its slot.
+    int Attach(WorldState *s)
+    {
+        std::string why = CanSupport(s);
+        if (!why.empty()) throw std::runtime_error("Can't support this world : " + why);
+        pdpd_host::WorldDescBuffers b(*s);
+        int slot = -1;
+        pdpd_host::check(pdpd_world_add(m_ctx.get(), &b.desc, &slot), "pdpd_world_add");
+        m_worlds.push_back({slot, s});
+        return slot;
+    }
+
+    void Detach(int slot)
+    {
+        for (size_t i = 0; i < m_worlds.size(); i++)
+            if (m_worlds[i].slot == slot) {
+                pdpd_host::check(pdpd_world_remove(m_ctx.get(), slot), "pdpd_world_remove");
+                m_worlds.erase(m_worlds.begin() + i);
+                return;
+            }
+        throw std::runtime_error("HipBatchDPDEngine: unknown slot");
+    }
+
+    // Step budget of one world for the next Run/Enqueue (default: all steps).
+    void SetSteps(int slot, unsigned steps) { pdpd_host::check(pdpd_world_set_steps(m_ctx.get(), slot, steps), "set_steps"); }
+
+    // Advance every attached world and synchronise all WorldStates.
+    void Run(unsigned nSteps)
+    {
+        Enqueue(nSteps);
+        for (auto &w : m_worlds) pdpd_host::download_into(m_ctx.get(), w.slot, *w.state);
+    }
+
+    // Asynchronous advance; Sync(slot) brings one world's WorldState up to date.
+    void Enqueue(unsigned nSteps) { pdpd_host::check(pdpd_run(m_ctx.get(), nSteps), "pdpd_run"); }
+    void Sync(int slot)
+    {
+        for (auto &w : m_worlds)
+            if (w.slot == slot) { pdpd_host::download_into(m_ctx.get(), slot, *w.state); return; }
+        throw std::runtime_error("HipBatchDPDEngine: unknown slot");
+    }
+
+    // Let the autotuner pick the brick and launch shape for the attached worlds.
+    std::string Autotune(unsigned steps_per_trial)
+    {
+        char buf[256] = {0};
+        pdpd_host::check(pdpd_autotune(m_ctx.get(), steps_per_trial, buf, sizeof(buf)), "pdpd_autotune");
+        return buf;
+    }
+
+    void SetConfigEntry(const std::string &key, const std::string &value)
+    {
+        if (pdpd_set_config(m_ctx.get(), key.c_str(), value.c_str()) != PDPD_OK)
+            throw std::runtime_error(pdpd_last_error());
+    }
+
+    pdpd_stats Stats()
+    {
+        pdpd_stats st;
+        pdpd_host::check(pdpd_get_stats(m_ctx.get(), &st), "pdpd_get_stats");
+        return st;
+    }
+
+    size_t size() const { return m_worlds.size(); }
+
+private:
+    struct Entry { int slot; WorldState *state; };
+    pdpd_host::Context m_ctx;
+    std::vector<Entry> m_worlds;
+};

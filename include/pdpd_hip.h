@@ -1,0 +1,237 @@
+/*
+ * pdpd_hip.h -- C ABI of the MI355X (gfx950) DPD engine library libpdpd_hip.so.
+ *
+ * The reference has no foreign-function boundary: its engines are C++ classes
+ * behind DPDEngine (include/dpd/core/dpd_engine.hpp:10-216 in poets-dpd). This
+ * ABI is the thin layer the repository's C++ DPDEngine subclasses
+ * (include/pdpd/hip_dpd_engine.hpp) call; each entry point below names the
+ * reference interface whose work it carries. Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns PDPD_OK (0) or a negative PDPD_ERR_* code; the
+ *     message of the last failure on the calling thread is pdpd_last_error().
+ *   - A context owns one HIP device and one stream. It holds any number of
+ *     resident worlds ("slots"); pdpd_run advances all of them together in one
+ *     set of launches per step (the plain engine is a context with one world).
+ *   - World data crosses the ABI in the reference's WorldState layout
+ *     (dpd_state.hpp:141-264): FP64 x/v/f in bead-index order plus the polymer
+ *     topology. On the device the state is FP32 SoA sorted by cell.
+ *   - Calls on one context must not overlap; different contexts are independent.
+ */
+#ifndef PDPD_HIP_H
+#define PDPD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDPD_ABI_VERSION 1
+
+enum {
+    PDPD_OK = 0,
+    PDPD_ERR_INVALID = -1,      /* bad argument or unknown slot / key            */
+    PDPD_ERR_UNSUPPORTED = -2,  /* world the engine cannot run (see can_support)  */
+    PDPD_ERR_HIP = -3,          /* HIP runtime failure                            */
+    PDPD_ERR_CAPACITY = -4,     /* a kernel flagged an occupancy overflow         */
+    PDPD_ERR_STATE = -5,        /* a kernel flagged a bead that left the box       */
+    PDPD_ERR_NOMEM = -6         /* device or host allocation failed               */
+};
+
+/* Error flags a step can raise (pdpd_stats.flags). */
+#define PDPD_FLAG_BAD_POSITION   0x1u   /* non-finite x, or a jump of more than one box */
+#define PDPD_FLAG_TILE_OVERFLOW  0x2u   /* a brick's halo exceeded the LDS tile         */
+
+/* Text of the last error on this thread ("" if none). */
+const char *pdpd_last_error(void);
+int pdpd_abi_version(void);
+
+/* One world, in the reference's WorldState terms (dpd_state.hpp:141-264).
+ * Arrays are read during pdpd_world_add only. */
+typedef struct pdpd_world_desc {
+    double box[3];                 /* WorldState::box, integer valued          */
+    double dt;                     /* WorldState::dt                           */
+    double lambda;                 /* WorldState::lambda (engine uses 0.5 form) */
+    uint64_t seed;                 /* WorldState::seed                         */
+    int64_t t;                     /* WorldState::t                            */
+
+    uint32_t n_bead_types;
+    const double *conservative;    /* n_bead_types^2, row-major (interactions[i].conservative) */
+    const double *dissipative;     /* n_bead_types^2 (interactions[i].dissipative = gamma)     */
+    const uint8_t *stationary;     /* n_bead_types or NULL (BeadType::stationary)             */
+
+    uint32_t n_beads;
+    const double *x, *v, *f;       /* 3*n_beads, xyz per bead, bead-index order */
+    const uint8_t *bead_type;      /* Bead::bead_type      */
+    const uint8_t *is_monomer;     /* Bead::is_monomer     */
+    const uint32_t *polymer_id;    /* Bead::polymer_id     */
+    const uint8_t *polymer_offset; /* Bead::polymer_offset */
+
+    uint32_t n_polymer_types;      /* PolymerType (dpd_state.hpp:212-224)           */
+    const uint32_t *pt_bond_start; /* n_polymer_types+1 -> bonds                    */
+    const uint32_t *bond_head;     /* Bond::bead_offset_head                        */
+    const uint32_t *bond_tail;     /* Bond::bead_offset_tail                        */
+    const double *bond_kappa;      /* Bond::kappa                                    */
+    const double *bond_r0;         /* Bond::r0                                       */
+    const uint32_t *pt_angle_start;/* n_polymer_types+1 -> bond pairs               */
+    const uint32_t *angle_head_bond;   /* BondPair::bond_offset_head                */
+    const uint32_t *angle_tail_bond;   /* BondPair::bond_offset_tail                */
+    const double *angle_kappa;     /* BondPair::kappa                                */
+    const double *angle_theta0;    /* BondPair::theta0                               */
+
+    uint32_t n_polymers;           /* Polymer (dpd_state.hpp:226-234)               */
+    const uint32_t *polymer_type;  /* Polymer::polymer_type                          */
+    const uint32_t *polymer_bead_start; /* n_polymers+1 -> polymer_bead_ids         */
+    const uint32_t *polymer_bead_ids;   /* Polymer::bead_ids                         */
+} pdpd_world_desc;
+
+typedef struct pdpd_limits {
+    uint32_t max_bead_types;      /* 16: the 4-bit type field of BeadHash (dpd_state.hpp:18-35) */
+    uint32_t min_box;             /* 3: smallest side for a 27-cell stencil without aliasing     */
+    uint32_t max_box;             /* largest side accepted                                       */
+    uint32_t max_polymer_length;  /* 127: the 7-bit offset field of BeadHash                     */
+    uint32_t tile_capacity;       /* beads one brick's halo may hold in LDS                      */
+    uint32_t max_bonds_per_bead;  /* bonds + angles touching one bead                            */
+} pdpd_limits;
+
+/* Capability / limit query (DPDEngine::CanSupport, dpd_engine.hpp:57-128). */
+int pdpd_get_limits(pdpd_limits *out);
+
+/* "" if the world can run, otherwise the reason, as CanSupport returns it.
+ * The pointer stays valid until the next call on this thread. */
+const char *pdpd_can_support(const pdpd_world_desc *w);
+
+typedef struct pdpd_ctx pdpd_ctx;
+
+/* Context = device + stream + resident worlds (DPDEngine instance state). */
+int pdpd_ctx_create(int device, pdpd_ctx **out);
+int pdpd_ctx_destroy(pdpd_ctx *ctx);
+
+/* Attach a world (DPDEngine::Attach, dpd_engine.hpp:171-177): imports AoS FP64
+ * into FP32 with the v -= f*dt/2 shift of dpd_engine_avx2_half_merge_tbb.hpp:524.
+ * The upload takes effect at the next pdpd_run. */
+int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *w, int *slot_out);
+
+/* Detach (Attach(nullptr)). Resident data of the slot is dropped. */
+int pdpd_world_remove(pdpd_ctx *ctx, int slot);
+
+/* Steps this world may advance in the next pdpd_run (default: all of them).
+ * A world whose budget is spent keeps its state while the others advance. */
+int pdpd_world_set_steps(pdpd_ctx *ctx, int slot, uint32_t steps);
+
+/* Advance all resident worlds by nsteps (DPDEngine::Run(nSteps),
+ * dpd_engine.hpp:179-185). Asynchronous: returns once the work is enqueued. */
+int pdpd_run(pdpd_ctx *ctx, uint32_t nsteps);
+
+/* Wait for enqueued work; reports kernel-raised flags as errors. */
+int pdpd_sync(pdpd_ctx *ctx);
+
+/* Synchronise and export one world in bead-index order (the end of Run:
+ * dpd_engine_avx2_half_merge_tbb.hpp:537-557, v += f*dt/2 in FP64).
+ * Any output pointer may be NULL. */
+int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t);
+
+/* Raw FP32 engine state of one world in bead-index order: x, the shifted
+ * velocity w = v - f*dt/2, f; and the device step counter. For parity tests. */
+int pdpd_world_download_f32(pdpd_ctx *ctx, int slot, float *x, float *w, float *f, uint32_t *t);
+
+/* Cell order of one world after the last step: order[k] = bead index at sorted
+ * position k; cell_start[c] for c in [0, Lx*Ly*Lz] (world-local). */
+int pdpd_world_download_cells(pdpd_ctx *ctx, int slot, int32_t *order, int32_t *cell_start);
+
+/* Engine configuration (DPDEngine::GetConfigEntries / SetConfigEntry,
+ * dpd_engine.hpp:150-169). Unknown keys fail with PDPD_ERR_INVALID.
+ * Keys: "brick" ("auto" or "BXxBYxBZ"), "graph_steps" (0 = no graph),
+ *       "persistent_blocks" (0 = one workgroup per brick). */
+int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value);
+int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen);
+/* Newline-separated "key=value\tdescription" list of all entries. */
+int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen);
+
+/* Timings and kernel statistics (DPDEngine::GetTimings, dpd_engine.hpp:38-46,145). */
+typedef struct pdpd_stats {
+    double upload_s;              /* host->device import of attached worlds (timings_t::configure) */
+    double run_s;                 /* wall time inside pdpd_run + pdpd_sync                        */
+    uint64_t steps;               /* steps enqueued                                               */
+    uint64_t bead_steps;          /* sum over worlds of beads x steps actually advanced           */
+    uint32_t flags;               /* PDPD_FLAG_* raised since the last reset                      */
+    uint32_t n_worlds;
+    uint64_t n_beads;
+    uint32_t n_bricks;            /* work items of the pair-force kernel                          */
+    uint32_t brick[3];            /* brick shape of the largest world                             */
+    /* average kernel durations (ms) of the last profiled run, 0 if none */
+    double k_integrate_ms, k_scan_ms, k_scatter_ms, k_reorder_ms, k_force_ms, k_bonds_ms;
+    uint64_t profiled_steps;
+} pdpd_stats;
+
+int pdpd_get_stats(pdpd_ctx *ctx, pdpd_stats *out);
+
+/* Run nsteps with direct launches and HIP events around every kernel on the
+ * context's stream; fills the k_*_ms averages of pdpd_stats. Synchronous. */
+int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps);
+
+/* Pick the brick shape and launch shape for the resident worlds by timing
+ * candidates for steps_per_trial steps each (state is saved and restored, so
+ * the worlds do not advance). Writes the choice to the "brick" configuration. */
+int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len);
+
+/* ---- serving: continuous batching over one context (no counterpart in the
+ * reference, whose engines run one world per Run) ----
+ * Requests queue in arrival order. At each interval boundary the loop retires
+ * finished requests, admits arrived ones while the resident set fits the HBM
+ * budget, and advances every resident request by one interval in shared
+ * launches; requests with fewer steps left stop early via their step budget. */
+typedef struct pdpd_server pdpd_server;
+
+typedef struct pdpd_server_config {
+    int device;
+    uint64_t hbm_budget_bytes;    /* admission budget for resident worlds, 0 = unlimited */
+    uint32_t interval_steps;      /* steps between admissions / callbacks (e.g. 500)     */
+    uint32_t max_resident;        /* 0 = unlimited                                       */
+    double time_scale;            /* arrival times are multiplied by this; 0 = all at once */
+} pdpd_server_config;
+
+/* Called after every interval for every resident request with its exported
+ * state (bead-index order). Return 0 to cancel the request. */
+typedef int (*pdpd_interval_fn)(void *user, uint64_t request, uint32_t steps_done, int64_t t, uint32_t n_beads,
+                                const double *x, const double *v, const double *f);
+
+typedef struct pdpd_request_result {
+    uint64_t id;
+    uint32_t n_beads;
+    uint32_t steps;
+    double arrival_s, admit_s, done_s, latency_s;   /* seconds from the start of pdpd_server_run */
+    int status;                                     /* 0 done, 1 queued, 2 resident, 3 cancelled  */
+} pdpd_request_result;
+
+typedef struct pdpd_server_stats {
+    uint64_t requests, completed, intervals, max_resident;
+    double makespan_s, gpu_busy_s;
+    double bead_steps, bead_steps_per_s;             /* sum of beads x steps over the makespan */
+    double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
+} pdpd_server_stats;
+
+int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out);
+int pdpd_server_destroy(pdpd_server *s);
+int pdpd_server_set_config(pdpd_server *s, const char *key, const char *value);
+int pdpd_server_set_callback(pdpd_server *s, pdpd_interval_fn fn, void *user);
+/* Queue a request; the world is copied. */
+int pdpd_server_submit(pdpd_server *s, const pdpd_world_desc *w, double arrival_s, uint32_t steps, uint64_t *id_out);
+/* Serve until every queued request has completed (arrivals against the wall clock). */
+int pdpd_server_run(pdpd_server *s);
+int pdpd_server_result(pdpd_server *s, uint64_t id, pdpd_request_result *out, double *x, double *v, double *f);
+int pdpd_server_get_stats(pdpd_server *s, pdpd_server_stats *out);
+
+/* ---- test hooks (no counterpart in the reference) ---- */
+
+/* hash_rng_sym / default_hash (hash.hpp:81-88, dpd_maths_core.hpp:55-79) on the
+ * device for n triples; outputs may be NULL. */
+int pdpd_debug_rng(int device, uint32_t n, const uint64_t *t_hash, const uint32_t *a,
+                   const uint32_t *b, uint32_t *u32_out, float *f32_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+/*
+ * oracle.h -- CPU restatement of the poets-dpd time step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+ * and only to compute expected results or time the CPU baseline. The product
+ * (poets-dpd_amd/) never links or calls it.
+ *
+ * Three restatements live here, each pinned by tests/test_oracle_*.py:
+ *
+ *   orc_hash_*      counter-based RNG of include/dpd/core/hash.hpp:20-32,61-88
+ *                   and default_hash of include/dpd/maths/dpd_maths_core.hpp:55-79.
+ *                   Pinned bit-exactly by tests/golden/rng_golden.npz (written by
+ *                   the reference's own hash.hpp compiled in the build container).
+ *
+ *   orc_naive_*     FP64 step of NaiveDPDEngine<false>
+ *                   (include/dpd/engines/naive/naive_dpd_engine.hpp:256-346,437-565,
+ *                   567-742,746-824). Pinned against single-step / multi-step
+ *                   fixtures written by the compiled reference engine
+ *                   (tests/golden/naive_*.npz) and against closed forms.
+ *
+ *   orc_f32_*       FP32 restatement of the algorithm the HIP engine runs:
+ *                   integrate (dpd_maths_core_half_step_raw.hpp:42-72,114-124 with
+ *                   the AVX2 half-kick shift of dpd_engine_avx2_half_merge_tbb.hpp:
+ *                   524,554), cell key + stable cell order, the full 27-cell pair
+ *                   force of dpd_maths_core_half_step_raw.hpp:129-216, Hookean bonds
+ *                   (naive_dpd_engine_half_merge.hpp:638-654) and angle bonds
+ *                   (dpd_maths_core_half_step.hpp:178-231). Pinned against the FP64
+ *                   naive restatement (within FP32 error) and the invariants in
+ *                   tests/test_oracle_f32.py.
+ *
+ *   orc_hm_run      multi-threaded FP64 restatement of NaiveDPDEngineHalfMergeTBB
+ *                   (naive_dpd_engine_half_merge_tbb.hpp:151-262): 13-cell forward
+ *                   stencil with Newton-3, 8-colour 2x2x2 waves on pthreads. Used as
+ *                   the CPU baseline; pinned against orc_naive (same physics, FP64).
+ */
+#ifndef PDPD_ORACLE_H
+#define PDPD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG (bit-exact) ---- */
+uint64_t orc_mix64_m2(uint64_t x);
+uint64_t orc_t_hash(uint32_t t, uint64_t seed);
+uint32_t orc_hash_rng_sym(uint64_t t_hash, uint32_t a, uint32_t b);
+float    orc_default_hash(uint64_t t_hash, uint32_t a, uint32_t b);
+double   orc_rand_sym_f64(uint64_t t_hash, uint32_t a, uint32_t b);
+uint32_t orc_bead_hash(uint32_t bead_type, int is_monomer, uint32_t polymer_id, uint32_t polymer_offset);
+
+/* Vectorised helpers for tests. */
+void orc_t_hash_n(int n, const uint32_t *t, const uint64_t *seed, uint64_t *out);
+void orc_hash_rng_sym_n(int n, const uint64_t *t_hash, const uint32_t *a, const uint32_t *b,
+                        uint32_t *out_u32, float *out_f32);
+
+/* ---- flat world (bead-index order, FP64) ----
+ * Topology is pre-flattened in the reference's iteration order: polymer p owns
+ * bonds [poly_bond_start[p], poly_bond_start[p+1]) and angles
+ * [poly_angle_start[p], poly_angle_start[p+1]); bead references are bead indices. */
+typedef struct orc_world {
+    int32_t n_beads;
+    double box[3];
+    double dt, lambda;
+    uint64_t seed;
+    int64_t t;
+    int32_t n_types;
+    const double *conservative;   /* n_types*n_types */
+    const double *dissipative;    /* n_types*n_types (gamma, not sqrt) */
+    const uint8_t *stationary;    /* n_types, may be NULL */
+    double *x, *v, *f;            /* 3*n_beads each, xyz per bead */
+    const uint32_t *hash;         /* n_beads */
+    const uint8_t *bead_type;     /* n_beads */
+    int32_t n_polymers;
+    const int32_t *poly_bond_start;   /* n_polymers+1 */
+    const int32_t *poly_angle_start;  /* n_polymers+1 */
+    const int32_t *bond_head, *bond_tail;
+    const double *bond_kappa, *bond_r0;
+    const int32_t *angle_head, *angle_mid, *angle_tail;
+    const double *angle_kappa, *angle_theta0;
+} orc_world;
+
+/* FP64 NaiveDPDEngine<false> restatement: nsteps full steps, in place. */
+int orc_naive_run(orc_world *w, int nsteps);
+
+/* Multi-threaded FP64 half-merge restatement (CPU baseline): nsteps in place. */
+int orc_hm_run(orc_world *w, int nsteps, int nthreads);
+
+/* ---- FP32 restatement of the HIP engine ----
+ * State in bead-index order, FP32:
+ *   x  positions in [0, L)
+ *   w  "shifted" velocity v - dt/2 f  (the engine's stored velocity form)
+ *   f  force of the last step
+ * One call advances one step exactly as the HIP engine does and reports
+ *   order[n]        beads sorted by (cell key, bead index)   (bit-exact target)
+ *   cell_start[C+1] segment starts in that order              (bit-exact target)
+ *   key[n]          cell key of each bead after the drift      (bit-exact target)
+ *   fabs_sum[n]     sum over pair terms of |F_ij| (tolerance scale for f)
+ * Any of the report pointers may be NULL. Returns 0, or -1 on a bead that left
+ * the box by more than one period (the engine flags the same condition). */
+typedef struct orc_f32_world {
+    int32_t n_beads;
+    int32_t L[3];
+    float dt;
+    uint64_t seed;
+    uint32_t t;
+    int32_t n_types;
+    const float *a;        /* n_types*n_types conservative */
+    const float *sg;       /* n_types*n_types sqrt(dissipative) */
+    float *x, *w, *f;      /* 3*n_beads */
+    const uint32_t *hash;
+    int32_t n_polymers;
+    const int32_t *poly_bond_start, *poly_angle_start;
+    const int32_t *bond_head, *bond_tail;
+    const float *bond_kappa, *bond_r0;
+    const int32_t *angle_head, *angle_mid, *angle_tail;
+    const float *angle_kappa, *angle_cos0, *angle_sin0;
+} orc_f32_world;
+
+int orc_f32_step(orc_f32_world *w, int32_t *order, int32_t *cell_start, int32_t *key, float *fabs_sum);
+
+/* Import of FP64 bead-order state into the engine's FP32 form
+ * (dpd_engine_avx2_half_merge_tbb.hpp:515-524): x, v, f rounded to float,
+ * w = fmaf(f, -dt/2, v); an x that rounds up to L is set to the largest float
+ * below L. */
+void orc_f32_import(int n, const double *x, const double *v, const double *f, const int32_t *L, float dt,
+                    float *x32, float *w32, float *f32);
+
+/* Pieces of the FP32 step, for finer-grained parity tests. */
+void orc_f32_integrate(orc_f32_world *w);                          /* drift + wrap, in place */
+int  orc_f32_cells(const orc_f32_world *w, int32_t *key, int32_t *order, int32_t *cell_start);
+void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, float *fabs_sum);
+void orc_f32_bonds(const orc_f32_world *w, float *f_inout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

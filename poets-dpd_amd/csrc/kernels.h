@@ -1,0 +1,33 @@
+// kernels.h -- launch interface of the step kernels (kernels.hip).
+#pragma once
+
+#include "device_types.h"
+
+namespace pdpd {
+
+// Everything one step needs to launch.
+struct StepLaunch {
+    DeviceState S;
+    int cur;                 // buffer (and counter copy) holding the state at step start
+    uint32_t n_beads;
+    uint32_t n_cells;
+    uint32_t n_worlds;
+    uint32_t n_bead_blocks;
+    uint32_t n_bricks;
+    uint32_t n_polys;
+    uint32_t force_blocks;   // 0 = one workgroup per brick, else persistent grid size
+};
+
+// Events recorded between the kernels of one step (profiling runs only):
+// [0] integrate [1] scan [2] scatter [3] reorder [4] force [5] bonds [6]
+struct KernelEvents {
+    hipEvent_t ev[7];
+};
+
+hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
+hipError_t launch_debug_rng(uint32_t n, const uint64_t *th, const uint32_t *a, const uint32_t *b, uint32_t *u32,
+                            float *f32, hipStream_t st);
+
+}  // namespace pdpd

@@ -1,0 +1,403 @@
+"""ctypes binding of libpdpd_hip.so (C ABI in include/pdpd_hip.h).
+
+The library must be built in-tree (poets-dpd_amd/lib/libpdpd_hip.so, see
+__graft_entry__.build); loading fails loudly otherwise -- there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libpdpd_hip.so")
+
+PDPD_OK = 0
+ERRORS = {-1: "invalid", -2: "unsupported", -3: "hip", -4: "capacity", -5: "state", -6: "nomem"}
+
+
+class PdpdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdpd error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("box", C.c_double * 3), ("dt", C.c_double), ("lambda_", C.c_double),
+        ("seed", C.c_uint64), ("t", C.c_int64),
+        ("n_bead_types", C.c_uint32), ("conservative", C.c_void_p), ("dissipative", C.c_void_p),
+        ("stationary", C.c_void_p),
+        ("n_beads", C.c_uint32), ("x", C.c_void_p), ("v", C.c_void_p), ("f", C.c_void_p),
+        ("bead_type", C.c_void_p), ("is_monomer", C.c_void_p), ("polymer_id", C.c_void_p),
+        ("polymer_offset", C.c_void_p),
+        ("n_polymer_types", C.c_uint32), ("pt_bond_start", C.c_void_p), ("bond_head", C.c_void_p),
+        ("bond_tail", C.c_void_p), ("bond_kappa", C.c_void_p), ("bond_r0", C.c_void_p),
+        ("pt_angle_start", C.c_void_p), ("angle_head_bond", C.c_void_p), ("angle_tail_bond", C.c_void_p),
+        ("angle_kappa", C.c_void_p), ("angle_theta0", C.c_void_p),
+        ("n_polymers", C.c_uint32), ("polymer_type", C.c_void_p), ("polymer_bead_start", C.c_void_p),
+        ("polymer_bead_ids", C.c_void_p),
+    ]
+
+
+class Limits(C.Structure):
+    _fields_ = [("max_bead_types", C.c_uint32), ("min_box", C.c_uint32), ("max_box", C.c_uint32),
+                ("max_polymer_length", C.c_uint32), ("tile_capacity", C.c_uint32),
+                ("max_bonds_per_bead", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("upload_s", C.c_double), ("run_s", C.c_double), ("steps", C.c_uint64),
+                ("bead_steps", C.c_uint64), ("flags", C.c_uint32), ("n_worlds", C.c_uint32),
+                ("n_beads", C.c_uint64), ("n_bricks", C.c_uint32), ("brick", C.c_uint32 * 3),
+                ("k_integrate_ms", C.c_double), ("k_scan_ms", C.c_double), ("k_scatter_ms", C.c_double),
+                ("k_reorder_ms", C.c_double), ("k_force_ms", C.c_double), ("k_bonds_ms", C.c_double),
+                ("profiled_steps", C.c_uint64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["brick"] = tuple(self.brick)
+        return d
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_uint64), ("interval_steps", C.c_uint32),
+                ("max_resident", C.c_uint32), ("time_scale", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("n_beads", C.c_uint32), ("steps", C.c_uint32), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double), ("latency_s", C.c_double), ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("completed", C.c_uint64), ("intervals", C.c_uint64),
+                ("max_resident", C.c_uint64), ("makespan_s", C.c_double), ("gpu_busy_s", C.c_double),
+                ("bead_steps", C.c_double), ("bead_steps_per_s", C.c_double), ("p50_latency_s", C.c_double),
+                ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("max_latency_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+INTERVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32,
+                          C.c_void_p, C.c_void_p, C.c_void_p)
+
+# Every symbol include/pdpd_hip.h declares, with its signature.
+SIGNATURES = {
+    "pdpd_last_error": (C.c_char_p, []),
+    "pdpd_abi_version": (C.c_int, []),
+    "pdpd_get_limits": (C.c_int, [C.POINTER(Limits)]),
+    "pdpd_can_support": (C.c_char_p, [C.POINTER(WorldDesc)]),
+    "pdpd_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pdpd_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "pdpd_world_add": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.POINTER(C.c_int)]),
+    "pdpd_world_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "pdpd_world_set_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
+    "pdpd_run": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pdpd_sync": (C.c_int, [C.c_void_p]),
+    "pdpd_world_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_int64)]),
+    "pdpd_world_download_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_uint32)]),
+    "pdpd_world_download_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "pdpd_set_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "pdpd_get_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+    "pdpd_list_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "pdpd_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "pdpd_run_profiled": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pdpd_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_size_t]),
+    "pdpd_debug_rng": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    "pdpd_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "pdpd_server_destroy": (C.c_int, [C.c_void_p]),
+    "pdpd_server_set_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "pdpd_server_set_callback": (C.c_int, [C.c_void_p, INTERVAL_FN, C.c_void_p]),
+    "pdpd_server_submit": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_double, C.c_uint32,
+                                     C.POINTER(C.c_uint64)]),
+    "pdpd_server_run": (C.c_int, [C.c_void_p]),
+    "pdpd_server_result": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestResult), C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "pdpd_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libpdpd_hip.so and bind every entry point; raises if missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PdpdError(-3, f"HIP engine library not built: {p} (run __graft_entry__.build())")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != PDPD_OK:
+        raise PdpdError(rc, load().pdpd_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+class _Keep:
+    """Holds the numpy arrays a WorldDesc points into."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def __call__(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.arrays.append(a)
+        return a
+
+
+def world_desc(world):
+    """Pack a World (pdpd_amd.world) into the C ABI struct; returns (desc, keepalive)."""
+    keep = _Keep()
+    d = WorldDesc()
+    for k in range(3):
+        d.box[k] = float(world.box[k])
+    d.dt = float(world.dt)
+    d.lambda_ = float(world.lambda_)
+    d.seed = int(world.seed) & 0xFFFFFFFFFFFFFFFF
+    d.t = int(world.t)
+    nt = world.n_types
+    d.n_bead_types = nt
+    d.conservative = _ptr(keep(world.conservative.reshape(-1), np.float64))
+    d.dissipative = _ptr(keep(world.dissipative.reshape(-1), np.float64))
+    d.stationary = _ptr(keep(world.stationary, np.uint8)) if world.stationary is not None else None
+    d.n_beads = world.n_beads
+    d.x = _ptr(keep(world.x.reshape(-1), np.float64))
+    d.v = _ptr(keep(world.v.reshape(-1), np.float64))
+    d.f = _ptr(keep(world.f.reshape(-1), np.float64))
+    d.bead_type = _ptr(keep(world.bead_type, np.uint8))
+    d.is_monomer = _ptr(keep(world.is_monomer, np.uint8))
+    d.polymer_id = _ptr(keep(world.polymer_id, np.uint32))
+    d.polymer_offset = _ptr(keep(world.polymer_offset, np.uint8))
+    pts = world.polymer_types
+    d.n_polymer_types = len(pts)
+    bs, bh, bt, bk, br = [0], [], [], [], []
+    as_, ah, at, ak, a0 = [0], [], [], [], []
+    for pt in pts:
+        for (h, t, k, r0) in pt.bonds:
+            bh.append(h); bt.append(t); bk.append(k); br.append(r0)
+        bs.append(len(bh))
+        for (hb, tb, k, th) in pt.bond_pairs:
+            ah.append(hb); at.append(tb); ak.append(k); a0.append(th)
+        as_.append(len(ah))
+    d.pt_bond_start = _ptr(keep(bs, np.uint32))
+    d.bond_head = _ptr(keep(bh, np.uint32))
+    d.bond_tail = _ptr(keep(bt, np.uint32))
+    d.bond_kappa = _ptr(keep(bk, np.float64))
+    d.bond_r0 = _ptr(keep(br, np.float64))
+    d.pt_angle_start = _ptr(keep(as_, np.uint32))
+    d.angle_head_bond = _ptr(keep(ah, np.uint32))
+    d.angle_tail_bond = _ptr(keep(at, np.uint32))
+    d.angle_kappa = _ptr(keep(ak, np.float64))
+    d.angle_theta0 = _ptr(keep(a0, np.float64))
+    d.n_polymers = len(world.polymer_type_of)
+    d.polymer_type = _ptr(keep(world.polymer_type_of, np.uint32))
+    d.polymer_bead_start = _ptr(keep(world.polymer_bead_start, np.uint32))
+    d.polymer_bead_ids = _ptr(keep(world.polymer_bead_ids, np.uint32))
+    return d, keep
+
+
+def can_support(world):
+    d, keep = world_desc(world)
+    return load().pdpd_can_support(C.byref(d)).decode()
+
+
+def limits():
+    lim = Limits()
+    check(load().pdpd_get_limits(C.byref(lim)))
+    return {k: getattr(lim, k) for k, _ in Limits._fields_}
+
+
+def debug_rng(t_hash, a, b, device=0):
+    t_hash = np.ascontiguousarray(t_hash, dtype=np.uint64)
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    n = t_hash.size
+    u = np.zeros(n, dtype=np.uint32)
+    f = np.zeros(n, dtype=np.float32)
+    check(load().pdpd_debug_rng(device, n, _ptr(t_hash), _ptr(a), _ptr(b), _ptr(u), _ptr(f)))
+    return u, f
+
+
+class Context:
+    """A device context holding resident worlds (one world = the plain engine)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.pdpd_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.worlds = {}
+
+    def close(self):
+        if self.h:
+            self.lib.pdpd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, world):
+        d, keep = world_desc(world)
+        slot = C.c_int()
+        check(self.lib.pdpd_world_add(self.h, C.byref(d), C.byref(slot)))
+        self.worlds[slot.value] = (world.n_beads, world.box.copy(), world.dt)
+        return slot.value
+
+    def remove(self, slot):
+        check(self.lib.pdpd_world_remove(self.h, slot))
+        self.worlds.pop(slot, None)
+
+    def set_steps(self, slot, steps):
+        check(self.lib.pdpd_world_set_steps(self.h, slot, int(steps)))
+
+    def run(self, nsteps):
+        check(self.lib.pdpd_run(self.h, int(nsteps)))
+
+    def sync(self):
+        check(self.lib.pdpd_sync(self.h))
+
+    def run_profiled(self, nsteps):
+        check(self.lib.pdpd_run_profiled(self.h, int(nsteps)))
+
+    def download(self, slot):
+        n = self.worlds[slot][0]
+        x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3))
+        t = C.c_int64()
+        check(self.lib.pdpd_world_download(self.h, slot, _ptr(x), _ptr(v), _ptr(f), C.byref(t)))
+        return x, v, f, t.value
+
+    def download_f32(self, slot):
+        n = self.worlds[slot][0]
+        x = np.zeros((n, 3), np.float32); w = np.zeros((n, 3), np.float32); f = np.zeros((n, 3), np.float32)
+        t = C.c_uint32()
+        check(self.lib.pdpd_world_download_f32(self.h, slot, _ptr(x), _ptr(w), _ptr(f), C.byref(t)))
+        return x, w, f, t.value
+
+    def download_cells(self, slot):
+        n, box, _ = self.worlds[slot]
+        nc = int(np.prod(box))
+        order = np.zeros(n, np.int32)
+        cs = np.zeros(nc + 1, np.int32)
+        check(self.lib.pdpd_world_download_cells(self.h, slot, _ptr(order), _ptr(cs)))
+        return order, cs
+
+    def set_config(self, key, value):
+        check(self.lib.pdpd_set_config(self.h, key.encode(), str(value).encode()))
+
+    def get_config(self, key):
+        buf = C.create_string_buffer(256)
+        check(self.lib.pdpd_get_config(self.h, key.encode(), buf, 256))
+        return buf.value.decode()
+
+    def list_config(self):
+        buf = C.create_string_buffer(4096)
+        check(self.lib.pdpd_list_config(self.h, buf, 4096))
+        return buf.value.decode()
+
+    def stats(self):
+        s = Stats()
+        check(self.lib.pdpd_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def autotune(self, steps_per_trial=20):
+        buf = C.create_string_buffer(256)
+        check(self.lib.pdpd_autotune(self.h, int(steps_per_trial), buf, 256))
+        return buf.value.decode()
+
+
+class Server:
+    """Continuous-batching serving loop (pdpd_server_* in the C ABI)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, interval_steps=500, max_resident=0, time_scale=1.0):
+        self.lib = load()
+        cfg = ServerConfig(device, int(hbm_budget_bytes), int(interval_steps), int(max_resident), float(time_scale))
+        h = C.c_void_p()
+        check(self.lib.pdpd_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+        self._cb = None
+
+    def close(self):
+        if self.h:
+            self.lib.pdpd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, key, value):
+        check(self.lib.pdpd_server_set_config(self.h, key.encode(), str(value).encode()))
+
+    def set_callback(self, fn):
+        """fn(request_id, steps_done, t, x, v, f) -> bool (False cancels)."""
+        if fn is None:
+            self._cb = None
+            check(self.lib.pdpd_server_set_callback(self.h, INTERVAL_FN(), None))
+            return
+
+        def tramp(user, rid, done, t, n, x, v, f):
+            def arr(p):
+                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n, 3)).copy()
+            return 1 if fn(int(rid), int(done), int(t), arr(x), arr(v), arr(f)) else 0
+        self._cb = INTERVAL_FN(tramp)
+        check(self.lib.pdpd_server_set_callback(self.h, self._cb, None))
+
+    def submit(self, world, arrival_s, steps):
+        d, keep = world_desc(world)
+        rid = C.c_uint64()
+        check(self.lib.pdpd_server_submit(self.h, C.byref(d), float(arrival_s), int(steps), C.byref(rid)))
+        self.sizes[rid.value] = world.n_beads
+        return rid.value
+
+    def run(self):
+        check(self.lib.pdpd_server_run(self.h))
+
+    def result(self, rid, with_state=True):
+        r = RequestResult()
+        n = self.sizes[rid]
+        x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3))
+        if with_state:
+            check(self.lib.pdpd_server_result(self.h, rid, C.byref(r), _ptr(x), _ptr(v), _ptr(f)))
+        else:
+            check(self.lib.pdpd_server_result(self.h, rid, C.byref(r), None, None, None))
+        res = {k: getattr(r, k) for k, _ in RequestResult._fields_}
+        if with_state:
+            res.update(x=x, v=v, f=f)
+        return res
+
+    def stats(self):
+        s = ServerStats()
+        check(self.lib.pdpd_server_get_stats(self.h, C.byref(s)))
+        return s.as_dict()

@@ -1,0 +1,116 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own code.
+
+Run in the build container, where the reference tree exists:
+    make -C oracle ref && python tests/golden/make_golden.py
+The reference is compiled from its headers by oracle/Makefile into
+oracle/_ref/libref_shim.so (hash.hpp, dpd_maths_core.hpp, dpd_state.hpp,
+NaiveDPDEngine<false>, NaiveDPDEngineHalfMerge). This script only calls that
+library and stores inputs and outputs; no reference source is copied.
+
+Fixtures
+  rng_golden.npz      get_t_hash(t, seed), hash_rng_sym / default_hash triples,
+                      BeadHash::construct cases
+  naive_<case>.npz    a world and its state after 1 and 10 steps of
+                      NaiveDPDEngine<false> (and NaiveDPDEngineHalfMerge where
+                      its multiple-of-4 restriction allows)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "poets-dpd_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+import oracle as O                                   # noqa: E402
+from pdpd_amd import world as W                      # noqa: E402
+from fixture_io import world_to_arrays               # noqa: E402
+
+
+def rng_fixture():
+    R = O.ref_lib()
+    rng = np.random.default_rng(20240601)
+    # (t, seed) -> t_hash: random, plus edges of the u32 step counter
+    t = np.concatenate([np.array([0, 1, 2, 0xFFFFFFFF, 0x80000000, 12345], np.uint32),
+                        rng.integers(0, 2**32, 250, dtype=np.uint32)])
+    seed = np.concatenate([np.array([0, 1, 2**64 - 1, 1, 42, 0xDEADBEEF], np.uint64),
+                           rng.integers(0, 2**64, 250, dtype=np.uint64)])
+    th = np.array([R.ref_get_t_hash(int(a), int(b)) for a, b in zip(t, seed)], np.uint64)
+    # (t_hash, a, b) -> u32 / float; includes a==b, a>b, extremes and real bead hashes
+    n = 1024
+    rth = rng.integers(0, 2**64, n, dtype=np.uint64)
+    a = rng.integers(0, 2**32, n, dtype=np.uint32)
+    b = rng.integers(0, 2**32, n, dtype=np.uint32)
+    a[:8] = [0, 0xFFFFFFFF, 5, 7, 0x8000000F, 0x18000000, 0x30000001, 0x1]
+    b[:8] = [0, 0xFFFFFFFF, 7, 5, 0x8000000F, 0x18000001, 0x20010002, 0xFFFFFFFF]
+    rth[:4] = [0, 2**64 - 1, 0x0123456789ABCDEF, th[0]]
+    u = np.array([R.ref_hash_rng_sym(int(x), int(y), int(z)) for x, y, z in zip(rth, a, b)], np.uint32)
+    f = np.array([R.ref_default_hash(int(x), int(y), int(z)) for x, y, z in zip(rth, a, b)], np.float32)
+    # BeadHash::construct
+    bt = rng.integers(0, 16, 64).astype(np.uint32)
+    mono = (rng.random(64) < 0.5).astype(np.uint32)
+    pid = np.where(mono == 1, rng.integers(0, 2**27, 64), rng.integers(0, 2**20, 64)).astype(np.uint32)
+    off = np.where(mono == 1, 0, rng.integers(0, 128, 64)).astype(np.uint32)
+    bh = np.array([R.ref_bead_hash(int(p), int(q), int(r), int(s)) for p, q, r, s in zip(bt, mono, pid, off)],
+                  np.uint32)
+    np.savez_compressed(os.path.join(HERE, "rng_golden.npz"), t=t, seed=seed, t_hash=th,
+                        pair_t_hash=rth, a=a, b=b, u32=u, f32=f,
+                        bh_type=bt, bh_monomer=mono, bh_polymer_id=pid, bh_offset=off, bh_hash=bh)
+
+
+def naive_cases():
+    rngA = np.random.default_rng(7)
+    return {
+        # water box, one type, edge length 4 (smallest the validator allows)
+        "water4": W.make_world((4, 4, 4), seed=11),
+        # non-cubic, non-multiple-of-4 box with chains (half-merge cannot run it)
+        "poly_6x7x8": W.make_world((6, 7, 8), bonded_fraction=0.2, seed=12, a_matrix=W.random_a_matrix(rngA)),
+        # cubic 8^3 with chains, runnable by the half-merge engine too
+        "poly8": W.make_world((8, 8, 8), bonded_fraction=0.1, seed=13, a_matrix=W.random_a_matrix(rngA)),
+        # dense start: 8 beads per cell in a 2x2x2 lattice of one region
+        "mesh4": _mesh_world(),
+    }
+
+
+def _mesh_world():
+    b = W.WorldBuilder((4, 4, 4), dt=0.01, seed=99)
+    b.add_bead_type("A")
+    b.set_interaction("A", "A", 25.0, 4.5)
+    m = b.add_polymer_type("A", ["A"])
+    rng = np.random.default_rng(5)
+    for cx in range(4):
+        for cy in range(4):
+            for cz in range(2):
+                for dx in (0.25, 0.75):
+                    for dy in (0.25, 0.75):
+                        for dz in (0.25, 0.75):
+                            b.add_polymer(m, [cx + dx, cy + dy, cz + dz], rng.standard_normal(3))
+    return b.build()
+
+
+def naive_fixtures():
+    for name, w in naive_cases().items():
+        out = world_to_arrays(w)
+        for steps in (1, 10):
+            x, v, f, t = O.ref_run(w, "naive", steps)
+            out[f"naive{steps}_x"], out[f"naive{steps}_v"], out[f"naive{steps}_f"] = x, v, f
+            out[f"naive{steps}_t"] = np.array(t)
+            try:
+                x, v, f, t = O.ref_run(w, "half_merge", steps)
+                out[f"hm{steps}_x"], out[f"hm{steps}_v"], out[f"hm{steps}_f"] = x, v, f
+            except NotImplementedError:
+                pass
+        np.savez_compressed(os.path.join(HERE, f"naive_{name}.npz"), **out)
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        sys.exit("build the reference shim first: make -C oracle ref")
+    rng_fixture()
+    naive_fixtures()
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))

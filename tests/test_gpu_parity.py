@@ -1,0 +1,236 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and with the
+reference's own engine outputs.
+
+Bars (stated per test):
+  RNG draws, cell keys, cell order, positions after the drift: bit-exact.
+  Single-step forces: within 1e-5 of sum_j |F_ij| per bead against the FP32
+  oracle (the engine and the oracle perform the same FP32 operations, so this
+  is met with equality in practice), and within FP32 error of the FP64
+  reference engine.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fixture_io import world_from_arrays
+from pdpd_amd import capi
+from pdpd_amd import world as W
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def run_gpu(world, steps, **cfg):
+    ctx = capi.Context(0)
+    for k, v in cfg.items():
+        ctx.set_config(k, v)
+    slot = ctx.add(world)
+    if steps:
+        ctx.run(steps)
+    ctx.sync()
+    return ctx, slot
+
+
+def small_worlds():
+    rngA = np.random.default_rng(7)
+    return {
+        "water4": W.make_world((4, 4, 4), seed=11),
+        "water3": W.make_world((3, 3, 3), seed=2),
+        "poly_6x7x8": W.make_world((6, 7, 8), bonded_fraction=0.2, seed=12, a_matrix=W.random_a_matrix(rngA)),
+        "poly_9x5x13": W.make_world((9, 5, 13), bonded_fraction=0.15, seed=14, a_matrix=W.random_a_matrix(rngA)),
+        "water16": W.config_water(seed=3),
+    }
+
+
+def test_rng_bit_exact_vs_reference_vectors(gpu_lib):
+    z = np.load(os.path.join(GOLDEN, "rng_golden.npz"))
+    _say("rng: 1024 golden triples")
+    u, f = capi.debug_rng(z["pair_t_hash"], z["a"], z["b"])
+    assert np.array_equal(u, z["u32"])
+    assert np.array_equal(f.view(np.uint32), z["f32"].view(np.uint32))
+    r = np.random.default_rng(1)
+    th = r.integers(0, 2**64, 100000, dtype=np.uint64)
+    a = r.integers(0, 2**32, 100000, dtype=np.uint32)
+    b = r.integers(0, 2**32, 100000, dtype=np.uint32)
+    u, f = capi.debug_rng(th, a, b)
+    uo, fo = O.hash_rng_sym(th, a, b)
+    assert np.array_equal(u, uo) and np.array_equal(f.view(np.uint32), fo.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", list(small_worlds().keys()))
+def test_one_step_vs_oracle_f32(gpu_lib, name):
+    w = small_worlds()[name]
+    _say(f"one step: {name} n={w.n_beads} box={tuple(w.box)}")
+    ctx, slot = run_gpu(w, 1)
+    x, ww, f, t = ctx.download_f32(slot)
+    order, cs = ctx.download_cells(slot)
+    ref = O.F32State(w).step(1)
+    assert t == (int(w.t) + 1)
+    # drift + wrap and the counting sort: bit-exact
+    assert np.array_equal(x.view(np.uint32), ref.x.view(np.uint32))
+    assert np.array_equal(ww.view(np.uint32), ref.w.view(np.uint32))
+    assert np.array_equal(cs, ref.cell_start)
+    assert np.array_equal(order, ref.order)
+    # forces: 1e-5 of the per-bead sum of pair-force magnitudes
+    tol = 1e-5 * (ref.fabs[:, None] + np.abs(ref.f) + 1e-3)
+    assert np.all(np.abs(f - ref.f) <= tol)
+    _say(f"  bitwise-equal force components: {np.mean(f.view(np.uint32) == ref.f.view(np.uint32)):.4f}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["poly_6x7x8", "water16"])
+def test_multi_step_vs_oracle_f32(gpu_lib, name):
+    w = small_worlds()[name]
+    _say(f"20 steps: {name}")
+    ctx, slot = run_gpu(w, 20)
+    x, ww, f, t = ctx.download_f32(slot)
+    ref = O.F32State(w).step(20)
+    # identical FP32 operations: the trajectories agree to the last bit while the
+    # force sums do; allow FP32 noise growth in case they do not
+    dx = x.astype(np.float64) - ref.x
+    dx -= np.round(dx / w.box) * w.box
+    assert np.abs(dx).max() < 1e-4
+    assert np.abs(ww - ref.w).max() < 1e-3
+    assert np.abs(f - ref.f).max() < 1e-2 * max(1.0, np.abs(ref.f).max()) * 1e-1
+    _say(f"  bitwise x: {np.array_equal(x.view(np.uint32), ref.x.view(np.uint32))}")
+    ctx.close()
+
+
+CASES = sorted(f for f in os.listdir(GOLDEN) if f.startswith("naive_"))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_vs_reference_naive_engine(gpu_lib, case):
+    """Against NaiveDPDEngine<false> (FP64) outputs written by the reference.
+    Stated error for FP32 vs FP64 after s steps: |dx| <= 1e-4 sqrt(s),
+    |dv| <= 1e-3 sqrt(s), |df| <= 2e-5 * max|f| * sqrt(s) + 1e-3 (the
+    reference's own differential test allows 0.01 sqrt(s) for x and v)."""
+    z = np.load(os.path.join(GOLDEN, case))
+    w = world_from_arrays(z)
+    for steps in (1, 10):
+        _say(f"{case}: {steps} steps vs reference naive")
+        ctx, slot = run_gpu(w, steps)
+        x, v, f, t = ctx.download(slot)
+        assert t == int(z[f"naive{steps}_t"])
+        rx, rv, rf = z[f"naive{steps}_x"], z[f"naive{steps}_v"], z[f"naive{steps}_f"]
+        dx = x - rx
+        dx -= np.round(dx / w.box) * w.box
+        s = np.sqrt(steps)
+        assert np.abs(dx).max() <= 1e-4 * s
+        assert np.abs(v - rv).max() <= 1e-3 * s
+        assert np.abs(f - rf).max() <= 2e-5 * np.abs(rf).max() * s + 1e-3
+        ctx.close()
+
+
+def test_batch_alone_equals_in_mix_bitwise(gpu_lib):
+    """Each world's result is the same alone and inside a batch of uneven worlds
+    (same kernels, per-bead summation order independent of the batch)."""
+    rng = np.random.default_rng(5)
+    worlds = [W.make_world(tuple(int(v) for v in rng.integers(4, 12, 3)), bonded_fraction=float(rng.uniform(0, 0.2)),
+                           seed=100 + i, a_matrix=W.random_a_matrix(rng)) for i in range(6)]
+    _say("batch of 6 vs alone, 15 steps")
+    ctx = capi.Context(0)
+    slots = [ctx.add(w) for w in worlds]
+    ctx.run(15)
+    ctx.sync()
+    mixed = [ctx.download_f32(s) for s in slots]
+    ctx.close()
+    for w, m in zip(worlds, mixed):
+        c2, s2 = run_gpu(w, 15)
+        alone = c2.download_f32(s2)
+        for a, b in zip(alone[:3], m[:3]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert alone[3] == m[3]
+        c2.close()
+
+
+def test_step_budgets_freeze_worlds(gpu_lib):
+    """A world whose step budget is spent keeps its state while others advance."""
+    wa = W.make_world((5, 6, 7), bonded_fraction=0.1, seed=31, a_matrix=W.random_a_matrix(np.random.default_rng(3)))
+    wb = W.make_world((8, 8, 8), seed=32)
+    ctx = capi.Context(0)
+    sa, sb = ctx.add(wa), ctx.add(wb)
+    ctx.set_steps(sa, 3)
+    ctx.run(10)
+    ctx.sync()
+    ga = ctx.download_f32(sa)
+    gb = ctx.download_f32(sb)
+    assert ga[3] == 3 and gb[3] == 10
+    ctx.close()
+    for w, g, n in ((wa, ga, 3), (wb, gb, 10)):
+        c2, s2 = run_gpu(w, n)
+        alone = c2.download_f32(s2)
+        for a, b in zip(alone[:3], g[:3]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        c2.close()
+
+
+@pytest.mark.parametrize("cfg", [dict(graph_steps=0), dict(graph_steps=8), dict(brick="1x1x1"),
+                                 dict(brick="2x2x4"), dict(brick="4x4x4", persistent_blocks=64)])
+def test_launch_shapes_bitwise_identical(gpu_lib, cfg):
+    w = W.make_world((12, 10, 9), bonded_fraction=0.1, seed=41, a_matrix=W.random_a_matrix(np.random.default_rng(4)))
+    _say(f"launch shape {cfg}")
+    c1, s1 = run_gpu(w, 24, graph_steps=0, brick="auto")
+    base = c1.download_f32(s1)
+    c1.close()
+    c2, s2 = run_gpu(w, 24, **cfg)
+    got = c2.download_f32(s2)
+    c2.close()
+    for a, b in zip(base[:3], got[:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_export_round_trip_and_momentum(gpu_lib):
+    """Attach/Run/export: t advances by n, positions stay in the box, total
+    momentum stays at its initial value to FP32 rounding (pair forces are
+    antisymmetric bit for bit, bonds conserve momentum)."""
+    w = W.config_polymer(seed=5, L=16)
+    p0 = w.v.sum(axis=0)
+    _say("polymer 16^3, 200 steps")
+    ctx, slot = run_gpu(w, 200)
+    x, v, f, t = ctx.download(slot)
+    assert t == 200
+    assert np.all(x >= 0) and np.all(x < w.box)
+    assert np.abs(v.sum(axis=0) - p0).max() < 1e-3 * w.n_beads ** 0.5
+    ke = 0.5 * (v ** 2).sum() / w.n_beads
+    assert 1.0 < ke < 2.5      # started at kT = 1 (1.5 per bead); DPD thermostat holds it near 1.5
+    ctx.close()
+
+
+def test_tile_overflow_is_reported(gpu_lib):
+    """More beads in one brick halo than the LDS tile holds: the kernel skips the
+    brick and the host reports PDPD_ERR_CAPACITY (the reference AVX2 engine
+    throws on cell overflow the same way, dpd_engine_avx2_half_merge_tbb.hpp:816-820)."""
+    b = W.WorldBuilder((4, 4, 4), dt=0.01, seed=1)
+    b.add_bead_type("A")
+    b.set_interaction("A", "A", 25.0, 4.5)
+    m = b.add_polymer_type("A", ["A"])
+    r = np.random.default_rng(0)
+    for i in range(1500):
+        b.add_polymer(m, 1.0 + 0.9 * r.random(3))
+    w = b.build()
+    ctx = capi.Context(0)
+    ctx.add(w)
+    with pytest.raises(capi.PdpdError) as e:
+        ctx.run(1)
+        ctx.sync()
+    assert e.value.code == -4
+    ctx.close()
+
+
+def test_unsupported_world_reason(gpu_lib):
+    w = W.make_world((4, 4, 4), seed=1)
+    w.stationary = np.array([1], np.uint8)
+    assert capi.can_support(w) == "Engine does not support stationary bead types."
+    ctx = capi.Context(0)
+    with pytest.raises(capi.PdpdError) as e:
+        ctx.add(w)
+    assert e.value.code == -2
+    ctx.close()
