@@ -51,6 +51,7 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(world, target_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O   # the checker's tree, used here only to time the CPU baseline
     out = {}
     if O.ref_available(fast=True):

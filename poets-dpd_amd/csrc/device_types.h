@@ -11,7 +11,8 @@ constexpr int kMaxTypes = 16;          // 4-bit type field of the bead hash
 constexpr int kForceThreads = 256;     // pair-force workgroup (4 wave64)
 constexpr int kMaxHaloCells = 512;     // (Bx+2)(By+2)(Bz+2) of one brick
 constexpr int kMaxInnerCells = 256;    // Bx*By*Bz of one brick
-constexpr int kTileBeads = 1280;       // beads of one brick's halo staged in LDS
+constexpr int kTileBeads = 1280;       // beads of one brick's halo staged in LDS (u16 indices)
+constexpr int kPairList = 16;          // accepted-pair list per lane in the pair-force kernel
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 8;
 constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan block

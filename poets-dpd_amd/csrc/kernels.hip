@@ -273,6 +273,7 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
     __shared__ float s_a[kMaxTypes * kMaxTypes];
     __shared__ float s_sg[kMaxTypes * kMaxTypes];
     __shared__ uint32_t s_wsum[4];
+    __shared__ uint16_t s_list[kPairList * kForceThreads];
 
     const int tid = threadIdx.x;
     const WorldCtr *__restrict__ ctr = S.ctr;   // this step's counters
@@ -361,7 +362,11 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
         }
         __syncthreads();
 
-        // 3. one lane per home bead, 27 cells as 9 contiguous z-rows of 3 cells
+        // 3. one lane per home bead, 27 cells as 9 contiguous z-rows of 3 cells.
+        //    Candidates within the cutoff are first collected into a per-lane
+        //    list in LDS, then evaluated: the expensive pair body runs once per
+        //    accepted pair instead of once per candidate of any lane in the wave.
+        //    The list keeps candidate order, so the sum order is unchanged.
         const WorldCtr wc = ctr[it.world];
         const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
         const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
@@ -382,6 +387,31 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
             const uint32_t hh = fbits(hp.w);
             const uint32_t trow = (hh >> 28) * kMaxTypes;
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+            uint32_t cnt = 0;
+            auto flush = [&]() {
+                for (uint32_t q = 0; q < cnt; q++) {
+                    const uint32_t m = s_list[q * kForceThreads + tid];
+                    const float4 op = s_pos[m];
+                    const float4 ov = s_vel[m];
+                    const float dx = hp.x - op.x, dy = hp.y - op.y, dz = hp.z - op.z;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float rr = sqrtf(r2);
+                    const float inv_r = 1.0f / rr;
+                    const float wr = 1.0f - rr;
+                    const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+                    const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
+                    const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+                    const uint32_t oh = fbits(op.w);
+                    const uint32_t ti = trow + (oh >> 28);
+                    const float sgw = s_sg[ti] * wr;
+                    const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
+                    const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                    fx = fmaf(ex, F, fx);
+                    fy = fmaf(ey, F, fy);
+                    fz = fmaf(ez, F, fz);
+                }
+                cnt = 0;
+            };
 #pragma unroll 1
             for (int r = 0; r < 9; r++) {
                 const int ddx = r / 3 - 1, ddy = r % 3 - 1;
@@ -392,24 +422,12 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
                     const float dx = hp.x - op.x, dy = hp.y - op.y, dz = hp.z - op.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     if (r2 < 1.0f && r2 >= kMinR2 && m != li) {
-                        const float4 ov = s_vel[m];
-                        const float rr = sqrtf(r2);
-                        const float inv_r = 1.0f / rr;
-                        const float wr = 1.0f - rr;
-                        const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
-                        const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
-                        const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                        const uint32_t oh = fbits(op.w);
-                        const uint32_t ti = trow + (oh >> 28);
-                        const float sgw = s_sg[ti] * wr;
-                        const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
-                        const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
-                        fx = fmaf(ex, F, fx);
-                        fy = fmaf(ey, F, fy);
-                        fz = fmaf(ez, F, fz);
+                        s_list[cnt * kForceThreads + tid] = (uint16_t)m;
+                        if (++cnt == kPairList) flush();
                     }
                 }
             }
+            flush();
             S.frc[nxt][s_cglob[hc] + off] = make_float4(fx, fy, fz, 0.0f);
         }
     }
