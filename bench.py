@@ -176,8 +176,7 @@ def main():
                 "valu_fp32_frac": n * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
                 "step_hbm_frac": n * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
                 "kernel_ms_per_step": {k[2:-3]: st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms",
-                                                               "k_reorder_ms", "k_force_ms", "k_bonds_ms")},
-                "brick": list(st["brick"])}
+                                                               "k_reorder_ms", "k_force_ms", "k_bonds_ms")}}
     x, v, f, t = ctx.download(slot)
     ctx.close()
 

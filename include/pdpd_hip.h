@@ -42,7 +42,7 @@ enum {
 
 /* Error flags a step can raise (pdpd_stats.flags). */
 #define PDPD_FLAG_BAD_POSITION   0x1u   /* non-finite x, or a jump of more than one box */
-#define PDPD_FLAG_TILE_OVERFLOW  0x2u   /* a brick's halo exceeded the LDS tile         */
+#define PDPD_FLAG_SCAN_TIMEOUT   0x2u   /* the cell scan look-back ran out of spins      */
 
 /* Text of the last error on this thread ("" if none). */
 const char *pdpd_last_error(void);
@@ -92,7 +92,7 @@ typedef struct pdpd_limits {
     uint32_t min_box;             /* 3: smallest side for a 27-cell stencil without aliasing     */
     uint32_t max_box;             /* largest side accepted                                       */
     uint32_t max_polymer_length;  /* 127: the 7-bit offset field of BeadHash                     */
-    uint32_t tile_capacity;       /* beads one brick's halo may hold in LDS                      */
+    uint32_t max_beads;           /* beads one context may hold (26-bit slots in the pair lists)  */
     uint32_t max_bonds_per_bead;  /* bonds + angles touching one bead                            */
 } pdpd_limits;
 
@@ -143,8 +143,8 @@ int pdpd_world_download_cells(pdpd_ctx *ctx, int slot, int32_t *order, int32_t *
 
 /* Engine configuration (DPDEngine::GetConfigEntries / SetConfigEntry,
  * dpd_engine.hpp:150-169). Unknown keys fail with PDPD_ERR_INVALID.
- * Keys: "brick" ("auto" or "BXxBYxBZ"), "graph_steps" (0 = no graph),
- *       "persistent_blocks" (0 = one workgroup per brick). */
+ * Keys: "graph_steps" (steps per replayed hipGraph, even; 0 = direct launches),
+ *       "force_grid" (pair-force grid-stride size; 0 = one workgroup per 64 beads). */
 int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value);
 int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen);
 /* Newline-separated "key=value\tdescription" list of all entries. */
@@ -159,8 +159,7 @@ typedef struct pdpd_stats {
     uint32_t flags;               /* PDPD_FLAG_* raised since the last reset                      */
     uint32_t n_worlds;
     uint64_t n_beads;
-    uint32_t n_bricks;            /* work items of the pair-force kernel                          */
-    uint32_t brick[3];            /* brick shape of the largest world                             */
+    uint32_t n_force_blocks;      /* work items of the pair-force kernel (64 beads each)          */
     /* average kernel durations (ms) of the last profiled run, 0 if none */
     double k_integrate_ms, k_scan_ms, k_scatter_ms, k_reorder_ms, k_force_ms, k_bonds_ms;
     uint64_t profiled_steps;
@@ -172,9 +171,9 @@ int pdpd_get_stats(pdpd_ctx *ctx, pdpd_stats *out);
  * context's stream; fills the k_*_ms averages of pdpd_stats. Synchronous. */
 int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps);
 
-/* Pick the brick shape and launch shape for the resident worlds by timing
+/* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
- * the worlds do not advance). Writes the choice to the "brick" configuration. */
+ * the worlds do not advance); the choice stays in the configuration. */
 int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len);
 
 /* ---- serving: continuous batching over one context (no counterpart in the

@@ -431,8 +431,10 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
         const float hvx = w->w[3 * i], hvy = w->w[3 * i + 1], hvz = w->w[3 * i + 2];
         const uint32_t hh = w->hash[i];
         const int htype = (int)(hh >> 28);
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fa = 0.0f;
-        for (int ddx = -1; ddx <= 1; ddx++)
+        /* one partial sum per x-plane of the stencil, combined (p0 + p1) + p2 */
+        float px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, pz[3] = {0, 0, 0}, fa = 0.0f;
+        for (int ddx = -1; ddx <= 1; ddx++) {
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (int ddy = -1; ddy <= 1; ddy++)
         for (int ddz = -1; ddz <= 1; ddz++) {
             int oc3[3];
@@ -472,9 +474,13 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
                 fa += fabsf(F);
             }
         }
-        f_out[3 * i] = fx;
-        f_out[3 * i + 1] = fy;
-        f_out[3 * i + 2] = fz;
+        px[ddx + 1] = fx;
+        py[ddx + 1] = fy;
+        pz[ddx + 1] = fz;
+        }
+        f_out[3 * i] = (px[0] + px[1]) + px[2];
+        f_out[3 * i + 1] = (py[0] + py[1]) + py[2];
+        f_out[3 * i + 2] = (pz[0] + pz[1]) + pz[2];
         if (fabs_sum) fabs_sum[i] = fa;
     }
     free(key);
@@ -533,7 +539,9 @@ void orc_f32_bonds(const orc_f32_world *w, float *F)
                 float fm = w->angle_kappa[ai] / mag;
                 float s0 = w->angle_sin0[ai];
                 if (fabsf(dot) > 0.000001f && s0 > 0.0f) {
-                    float ip = 1.0f / sqrtf(1.0f / cos2 - 1.0f);
+                    /* prefactor floor of naive_dpd_engine.hpp:681-682; argument clamped at 0 */
+                    float pref = fmaxf(sqrtf(fmaxf(1.0f / cos2 - 1.0f, 0.0f)), 0.000001f);
+                    float ip = 1.0f / pref;
                     fm = fm * (w->angle_cos0[ai] - s0 * ip);
                 }
                 for (int d = 0; d < 3; d++) {

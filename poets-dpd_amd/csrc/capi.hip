@@ -59,11 +59,6 @@ double now_s()
 constexpr uint32_t kMaxBox = 1024;
 constexpr uint32_t kMaxPolymerLength = 127;
 
-// Brick shapes tried by the heuristic and the autotuner, largest first.
-const int kBrickCandidates[][3] = {
-    {4, 4, 8}, {4, 4, 4}, {2, 4, 4}, {2, 2, 4}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1},
-};
-
 uint32_t bead_hash(uint32_t type, bool monomer, uint32_t polymer_id, uint32_t offset)
 {
     // dpd_state.hpp:52-60
@@ -86,14 +81,9 @@ struct WorldHost {
     float a[kMaxTypes * kMaxTypes] = {};
     float sg[kMaxTypes * kMaxTypes] = {};
     std::vector<float4> up_pos, up_vel, up_frc;   // pending upload (bead order)
-    std::vector<uint32_t> cell_occupancy;          // at import, for brick feasibility
-    std::vector<BondRec> bonds;                    // local bead indices
+    std::vector<BondRec> bonds;                    // local bead indices, reference order
     std::vector<AngleRec> angles;
-    struct Poly { uint32_t b0, b1, a0, a1; };
-    std::vector<Poly> polys;
-    // layout
-    uint32_t bead_off = 0, cell_off = 0;
-    int brick[3] = {4, 4, 4};
+    uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
 };
 
@@ -119,6 +109,14 @@ hipError_t dev_alloc(DevBuf<T> &b, size_t n)
     return e;
 }
 
+template <class T>
+hipError_t dev_upload(DevBuf<T> &b, const std::vector<T> &v)
+{
+    hipError_t e = dev_alloc(b, v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
 }  // namespace
 
 void pdpd_internal_set_error(const char *msg) { g_err = msg ? msg : ""; }
@@ -133,24 +131,23 @@ struct pdpd_ctx {
 
     // device layout
     DevBuf<float4> pos[2], vel[2], frc[2];
-    DevBuf<uint32_t> key, rank, perm, cell_count, cell_start, block_sums, slot_of_bead, flags;
+    DevBuf<uint32_t> key, rank, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
+    DevBuf<unsigned long long> scan_tiles;
     DevBuf<WorldDev> wdev;
     DevBuf<WorldCtr> ctr;
-    DevBuf<BrickItem> bricks;
-    DevBuf<BeadBlock> blocks;
-    DevBuf<PolyItem> polys;
+    DevBuf<BeadBlock> bblocks, fblocks;
+    DevBuf<BondedBead> bonded;
+    DevBuf<uint32_t> bead_terms;
     DevBuf<BondRec> bonds;
     DevBuf<AngleRec> angles;
     int cur = 0;
-    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bricks = 0, n_blocks = 0, n_polys = 0;
+    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_bonded = 0;
     std::vector<int> order;               // slots in layout order
     std::vector<WorldCtr> ctr_host;
 
     // configuration
-    std::string brick_cfg = "auto";
     uint32_t graph_steps = 20;
-    uint32_t persistent_blocks = 0;
-    int brick_used[3] = {0, 0, 0};
+    uint32_t force_grid = 0;
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -164,9 +161,10 @@ struct pdpd_ctx {
     {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
-        key.release(); rank.release(); perm.release(); cell_count.release(); cell_start.release();
-        block_sums.release(); slot_of_bead.release(); flags.release(); wdev.release(); ctr.release();
-        bricks.release(); blocks.release(); polys.release(); bonds.release(); angles.release();
+        key.release(); rank.release(); perm.release(); sorted_idx.release(); cell_count.release();
+        cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
+        wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); bonded.release(); bead_terms.release();
+        bonds.release(); angles.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -174,11 +172,12 @@ struct pdpd_ctx {
     {
         DeviceState S{};
         for (int b = 0; b < 2; b++) { S.pos[b] = pos[b].p; S.vel[b] = vel[b].p; S.frc[b] = frc[b].p; }
-        S.key = key.p; S.rank = rank.p; S.perm = perm.p;
-        S.cell_count = cell_count.p; S.cell_start = cell_start.p; S.block_sums = block_sums.p;
+        S.key = key.p; S.rank = rank.p; S.perm = perm.p; S.sorted_idx = sorted_idx.p;
+        S.cell_count = cell_count.p; S.cell_start = cell_start.p;
+        S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
         S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
-        S.worlds = wdev.p; S.ctr = ctr.p; S.bricks = bricks.p; S.bead_blocks = blocks.p;
-        S.polys = polys.p; S.bonds = bonds.p; S.angles = angles.p;
+        S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
+        S.bonded = bonded.p; S.bead_terms = bead_terms.p; S.bonds = bonds.p; S.angles = angles.p;
         return S;
     }
 
@@ -188,114 +187,44 @@ struct pdpd_ctx {
         L.S = state();
         L.cur = c;
         L.n_beads = n_beads; L.n_cells = n_cells; L.n_worlds = n_worlds;
-        L.n_bead_blocks = n_blocks; L.n_bricks = n_bricks; L.n_polys = n_polys;
-        L.force_blocks = persistent_blocks;
+        L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks; L.n_bonded = n_bonded;
+        L.force_grid = force_grid;
         return L;
     }
 };
 
 namespace {
 
-// Largest halo count over the bricks of a world for a brick shape, from the
-// occupancy recorded at import (or downloaded for resident worlds).
-uint32_t max_halo_count(const WorldHost &w, const int *B)
+// Work lists over all resident worlds. Force blocks of large worlds are
+// interleaved with those of small ones so a persistent grid sees an even mix.
+void build_blocks(const std::vector<const WorldHost *> &ws, std::vector<BeadBlock> &bb, std::vector<BeadBlock> &fb)
 {
-    const int L0 = w.L[0], L1 = w.L[1], L2 = w.L[2];
-    if (w.cell_occupancy.empty()) return 0;
-    uint32_t best = 0;
-    for (int bx = 0; bx < L0; bx += B[0])
-    for (int by = 0; by < L1; by += B[1])
-    for (int bz = 0; bz < L2; bz += B[2]) {
-        int n0 = std::min(B[0], L0 - bx), n1 = std::min(B[1], L1 - by), n2 = std::min(B[2], L2 - bz);
-        uint32_t s = 0;
-        for (int ix = -1; ix <= n0; ix++)
-        for (int iy = -1; iy <= n1; iy++)
-        for (int iz = -1; iz <= n2; iz++) {
-            int gx = (bx + ix + L0) % L0, gy = (by + iy + L1) % L1, gz = (bz + iz + L2) % L2;
-            s += w.cell_occupancy[(size_t)(gx * L1 + gy) * L2 + gz];
+    bb.clear();
+    fb.clear();
+    std::vector<std::vector<BeadBlock>> per(ws.size());
+    for (size_t i = 0; i < ws.size(); i++) {
+        const WorldHost &w = *ws[i];
+        for (uint32_t b = 0; b < w.n; b += kBeadThreads)
+            bb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kBeadThreads), 0});
+        for (uint32_t b = 0; b < w.n; b += kForceBeads)
+            per[i].push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kForceBeads), 0});
+    }
+    // round-robin by world, largest first
+    std::vector<size_t> idx(ws.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return per[a].size() > per[b].size(); });
+    size_t round = 0;
+    bool any = true;
+    while (any) {
+        any = false;
+        for (size_t i : idx) {
+            if (round < per[i].size()) {
+                fb.push_back(per[i][round]);
+                any = true;
+            }
         }
-        best = std::max(best, s);
+        round++;
     }
-    return best;
-}
-
-bool brick_fits_lds(const int *B)
-{
-    return (B[0] + 2) * (B[1] + 2) * (B[2] + 2) <= kMaxHaloCells && B[0] * B[1] * B[2] <= kMaxInnerCells;
-}
-
-uint64_t count_bricks(const WorldHost &w, const int *B)
-{
-    return (uint64_t)((w.L[0] + B[0] - 1) / B[0]) * ((w.L[1] + B[1] - 1) / B[1]) * ((w.L[2] + B[2] - 1) / B[2]);
-}
-
-bool parse_brick(const std::string &s, int *B)
-{
-    int a, b, c;
-    char x1, x2;
-    if (sscanf(s.c_str(), "%d%c%d%c%d", &a, &x1, &b, &x2, &c) != 5) return false;
-    if (x1 != 'x' || x2 != 'x' || a < 1 || b < 1 || c < 1) return false;
-    B[0] = a; B[1] = b; B[2] = c;
-    return brick_fits_lds(B);
-}
-
-// Choose one brick shape for the layout: an explicit configuration, or the
-// largest candidate whose halo fits the tile with 25% headroom in every world
-// and that still gives at least 1024 work items.
-void choose_bricks(pdpd_ctx *ctx)
-{
-    int B[3];
-    if (ctx->brick_cfg != "auto" && parse_brick(ctx->brick_cfg, B)) {
-        for (auto &kv : ctx->worlds) std::copy(B, B + 3, kv.second.brick);
-        std::copy(B, B + 3, ctx->brick_used);
-        return;
-    }
-    const int *pick = nullptr;
-    for (const auto &cand : kBrickCandidates) {
-        bool ok = brick_fits_lds(cand);
-        uint64_t items = 0;
-        for (auto &kv : ctx->worlds) {
-            if (!ok) break;
-            uint32_t h = max_halo_count(kv.second, cand);
-            ok = h * 5 / 4 + 16 <= (uint32_t)kTileBeads;
-            items += count_bricks(kv.second, cand);
-        }
-        if (!ok) continue;
-        pick = cand;
-        if (items >= 1024) break;
-    }
-    static const int one[3] = {1, 1, 1};
-    if (!pick) pick = one;
-    for (auto &kv : ctx->worlds) std::copy(pick, pick + 3, kv.second.brick);
-    std::copy(pick, pick + 3, ctx->brick_used);
-}
-
-int build_brick_table(pdpd_ctx *ctx)
-{
-    std::vector<BrickItem> items;
-    for (size_t wi = 0; wi < ctx->order.size(); wi++) {
-        const WorldHost &w = ctx->worlds.at(ctx->order[wi]);
-        for (int bx = 0; bx < w.L[0]; bx += w.brick[0])
-        for (int by = 0; by < w.L[1]; by += w.brick[1])
-        for (int bz = 0; bz < w.L[2]; bz += w.brick[2]) items.push_back({(int32_t)wi, bx, by, bz});
-    }
-    // interleave worlds so a persistent grid sees mixed sizes evenly
-    ctx->n_bricks = (uint32_t)items.size();
-    if (ctx->bricks.n < items.size()) HIP_TRY(dev_alloc(ctx->bricks, items.size()));
-    if (!items.empty())
-        HIP_TRY(hipMemcpy(ctx->bricks.p, items.data(), items.size() * sizeof(BrickItem), hipMemcpyHostToDevice));
-    // world table carries the brick shape
-    std::vector<WorldDev> wd(ctx->order.size());
-    HIP_TRY(hipMemcpy(wd.data(), ctx->wdev.p, wd.size() * sizeof(WorldDev), hipMemcpyDeviceToHost));
-    for (size_t wi = 0; wi < ctx->order.size(); wi++) {
-        const WorldHost &w = ctx->worlds.at(ctx->order[wi]);
-        for (int d = 0; d < 3; d++) wd[wi].brick[d] = w.brick[d];
-    }
-    HIP_TRY(hipMemcpy(ctx->wdev.p, wd.data(), wd.size() * sizeof(WorldDev), hipMemcpyHostToDevice));
-    ctx->stats.n_bricks = ctx->n_bricks;
-    std::copy(ctx->brick_used, ctx->brick_used + 3, ctx->stats.brick);
-    ctx->graph_version = ~0ull;
-    return PDPD_OK;
 }
 
 // Rebuild the device layout: resident worlds are moved to their new offsets,
@@ -316,9 +245,9 @@ int relayout(pdpd_ctx *ctx)
         nb += w.n;
         nc += w.n_cells();
     }
-    if (nb >= 0x7fffffffull || nc >= 0x7fffffffull) return fail(PDPD_ERR_INVALID, "layout exceeds 2^31 beads or cells");
+    if (nb >= kMaxLayoutBeads) return fail(PDPD_ERR_INVALID, "resident worlds exceed 2^26 beads in one context");
+    if (nc >= 0x7fffffffull) return fail(PDPD_ERR_INVALID, "resident worlds exceed 2^31 cells in one context");
 
-    // new bead buffers
     DevBuf<float4> npos[2], nvel[2], nfrc[2];
     for (int b = 0; b < 2; b++) {
         HIP_TRY(dev_alloc(npos[b], nb));
@@ -352,19 +281,20 @@ int relayout(pdpd_ctx *ctx)
     for (int b = 0; b < 2; b++) {
         ctx->pos[b].release(); ctx->vel[b].release(); ctx->frc[b].release();
         ctx->pos[b] = npos[b]; ctx->vel[b] = nvel[b]; ctx->frc[b] = nfrc[b];
-        npos[b].p = nullptr; nvel[b].p = nullptr; nfrc[b].p = nullptr;
     }
     ctx->cur = 0;
+    std::vector<const WorldHost *> ws;
     for (size_t i = 0; i < order.size(); i++) {
         WorldHost &w = ctx->worlds.at(order[i]);
         w.bead_off = new_bead_off[i];
         w.cell_off = new_cell_off[i];
         if (!w.resident) {
-            w.up_pos.clear(); w.up_pos.shrink_to_fit();
-            w.up_vel.clear(); w.up_vel.shrink_to_fit();
-            w.up_frc.clear(); w.up_frc.shrink_to_fit();
+            std::vector<float4>().swap(w.up_pos);
+            std::vector<float4>().swap(w.up_vel);
+            std::vector<float4>().swap(w.up_frc);
             w.resident = true;
         }
+        ws.push_back(&w);
     }
     ctx->order = order;
     ctx->n_beads = (uint32_t)nb;
@@ -374,23 +304,25 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_alloc(ctx->key, nb));
     HIP_TRY(dev_alloc(ctx->rank, nb));
     HIP_TRY(dev_alloc(ctx->perm, nb));
+    HIP_TRY(dev_alloc(ctx->sorted_idx, nb));
     HIP_TRY(dev_alloc(ctx->slot_of_bead, nb));
     HIP_TRY(dev_alloc(ctx->cell_count, nc));
     HIP_TRY(dev_alloc(ctx->cell_start, nc + 1));
-    HIP_TRY(dev_alloc(ctx->block_sums, (nc + kScanBlock - 1) / kScanBlock + 1));
+    HIP_TRY(dev_alloc(ctx->scan_tiles, scan_tiles((uint32_t)nc)));
     if (!ctx->flags.p) {
         HIP_TRY(dev_alloc(ctx->flags, 1));
         HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
     }
+    if (!ctx->scan_ticket.p) HIP_TRY(dev_alloc(ctx->scan_ticket, 1));
     HIP_TRY(launch_fill_u32(ctx->cell_count.p, (uint32_t)nc, 0, ctx->stream));
 
     // world table
     std::vector<WorldDev> wd(order.size());
     for (size_t i = 0; i < order.size(); i++) {
-        const WorldHost &w = ctx->worlds.at(order[i]);
+        const WorldHost &w = *ws[i];
         WorldDev &d = wd[i];
         memset(&d, 0, sizeof(d));
-        for (int k = 0; k < 3; k++) { d.L[k] = w.L[k]; d.Lf[k] = (float)w.L[k]; d.brick[k] = w.brick[k]; }
+        for (int k = 0; k < 3; k++) { d.L[k] = w.L[k]; d.Lf[k] = (float)w.L[k]; }
         d.dt = (float)w.dt;
         d.hdt = 0.5f * d.dt;
         d.rng_scale = (float)std::sqrt(24.0 / (double)d.dt);
@@ -401,66 +333,61 @@ int relayout(pdpd_ctx *ctx)
         memcpy(d.a, w.a, sizeof(d.a));
         memcpy(d.sg, w.sg, sizeof(d.sg));
     }
-    HIP_TRY(dev_alloc(ctx->wdev, wd.size()));
-    if (!wd.empty()) HIP_TRY(hipMemcpy(ctx->wdev.p, wd.data(), wd.size() * sizeof(WorldDev), hipMemcpyHostToDevice));
+    HIP_TRY(dev_upload(ctx->wdev, wd));
     HIP_TRY(dev_alloc(ctx->ctr, 2 * std::max<size_t>(1, order.size())));
     ctx->ctr_host.assign(2 * std::max<size_t>(1, order.size()), WorldCtr{0, 0});
 
-    // bead blocks
-    std::vector<BeadBlock> blocks;
-    for (size_t i = 0; i < order.size(); i++) {
-        const WorldHost &w = ctx->worlds.at(order[i]);
-        for (uint32_t b = 0; b < w.n; b += kBeadThreads)
-            blocks.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kBeadThreads), 0});
-    }
-    ctx->n_blocks = (uint32_t)blocks.size();
-    HIP_TRY(dev_alloc(ctx->blocks, blocks.size()));
-    if (!blocks.empty())
-        HIP_TRY(hipMemcpy(ctx->blocks.p, blocks.data(), blocks.size() * sizeof(BeadBlock), hipMemcpyHostToDevice));
+    std::vector<BeadBlock> bb, fb;
+    build_blocks(ws, bb, fb);
+    ctx->n_bblocks = (uint32_t)bb.size();
+    ctx->n_fblocks = (uint32_t)fb.size();
+    HIP_TRY(dev_upload(ctx->bblocks, bb));
+    HIP_TRY(dev_upload(ctx->fblocks, fb));
 
-    // bonded topology with global bead indices
-    std::vector<PolyItem> pitems;
+    // bonded topology with global bead indices, and per bonded bead its terms in
+    // the order the per-polymer loop applies them
     std::vector<BondRec> brec;
     std::vector<AngleRec> arec;
-    for (size_t i = 0; i < order.size(); i++) {
-        const WorldHost &w = ctx->worlds.at(order[i]);
-        for (const auto &p : w.polys) {
-            PolyItem it{};
-            it.world = (int32_t)i;
-            it.bond_begin = (uint32_t)brec.size();
-            for (uint32_t k = p.b0; k < p.b1; k++) {
-                BondRec r = w.bonds[k];
-                r.head += w.bead_off; r.tail += w.bead_off;
-                brec.push_back(r);
-            }
-            it.bond_end = (uint32_t)brec.size();
-            it.angle_begin = (uint32_t)arec.size();
-            for (uint32_t k = p.a0; k < p.a1; k++) {
-                AngleRec r = w.angles[k];
-                r.head += w.bead_off; r.mid += w.bead_off; r.tail += w.bead_off;
-                arec.push_back(r);
-            }
-            it.angle_end = (uint32_t)arec.size();
-            pitems.push_back(it);
+    std::vector<BondedBead> bbead;
+    std::vector<uint32_t> terms;
+    for (size_t i = 0; i < ws.size(); i++) {
+        const WorldHost &w = *ws[i];
+        if (w.bonds.empty() && w.angles.empty()) continue;
+        std::vector<std::vector<uint32_t>> per(w.n);
+        const uint32_t b0 = (uint32_t)brec.size(), a0 = (uint32_t)arec.size();
+        for (size_t k = 0; k < w.bonds.size(); k++) {
+            BondRec r = w.bonds[k];
+            per[r.head].push_back((kRoleBondHead << kRoleShift) | (b0 + (uint32_t)k));
+            per[r.tail].push_back((kRoleBondTail << kRoleShift) | (b0 + (uint32_t)k));
+            r.head += w.bead_off; r.tail += w.bead_off;
+            brec.push_back(r);
+        }
+        for (size_t k = 0; k < w.angles.size(); k++) {
+            AngleRec r = w.angles[k];
+            per[r.head].push_back((kRoleAngleHead << kRoleShift) | (a0 + (uint32_t)k));
+            per[r.tail].push_back((kRoleAngleTail << kRoleShift) | (a0 + (uint32_t)k));
+            per[r.mid].push_back((kRoleAngleMid << kRoleShift) | (a0 + (uint32_t)k));
+            r.head += w.bead_off; r.mid += w.bead_off; r.tail += w.bead_off;
+            arec.push_back(r);
+        }
+        for (uint32_t b = 0; b < w.n; b++) {
+            if (per[b].empty()) continue;
+            BondedBead e{w.bead_off + b, (uint32_t)terms.size(), 0, (int32_t)i};
+            terms.insert(terms.end(), per[b].begin(), per[b].end());
+            e.t1 = (uint32_t)terms.size();
+            bbead.push_back(e);
         }
     }
-    ctx->n_polys = (uint32_t)pitems.size();
-    HIP_TRY(dev_alloc(ctx->polys, pitems.size()));
-    HIP_TRY(dev_alloc(ctx->bonds, brec.size()));
-    HIP_TRY(dev_alloc(ctx->angles, arec.size()));
-    if (!pitems.empty())
-        HIP_TRY(hipMemcpy(ctx->polys.p, pitems.data(), pitems.size() * sizeof(PolyItem), hipMemcpyHostToDevice));
-    if (!brec.empty())
-        HIP_TRY(hipMemcpy(ctx->bonds.p, brec.data(), brec.size() * sizeof(BondRec), hipMemcpyHostToDevice));
-    if (!arec.empty())
-        HIP_TRY(hipMemcpy(ctx->angles.p, arec.data(), arec.size() * sizeof(AngleRec), hipMemcpyHostToDevice));
-
-    choose_bricks(ctx);
-    int rc = build_brick_table(ctx);
-    if (rc) return rc;
+    if (brec.size() + arec.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
+    ctx->n_bonded = (uint32_t)bbead.size();
+    HIP_TRY(dev_upload(ctx->bonds, brec));
+    HIP_TRY(dev_upload(ctx->angles, arec));
+    HIP_TRY(dev_upload(ctx->bonded, bbead));
+    HIP_TRY(dev_upload(ctx->bead_terms, terms));
 
     ctx->stats.n_worlds = ctx->n_worlds;
     ctx->stats.n_beads = ctx->n_beads;
+    ctx->stats.n_force_blocks = ctx->n_fblocks;
     ctx->layout_version++;
     ctx->graph_version = ~0ull;
     ctx->dirty = false;
@@ -525,7 +452,7 @@ int ensure_graph(pdpd_ctx *ctx, uint32_t len)
     for (uint32_t s = 0; s < len; s++) {
         hipError_t e = launch_step(ctx->step_launch(c), ctx->stream, nullptr);
         if (e != hipSuccess) {
-            hipGraph_t junk;
+            hipGraph_t junk = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &junk);
             if (junk) (void)hipGraphDestroy(junk);
             return fail(PDPD_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
@@ -542,6 +469,23 @@ int ensure_graph(pdpd_ctx *ctx, uint32_t len)
     return PDPD_OK;
 }
 
+// Enqueue nsteps: whole graphs of graph_steps (even, so buffer parity returns)
+// then single steps.
+int enqueue_steps(pdpd_ctx *ctx, uint32_t nsteps)
+{
+    uint32_t left = nsteps;
+    const uint32_t G = ctx->graph_steps & ~1u;
+    if (G >= 2 && left >= G) {
+        int rc = ensure_graph(ctx, G);
+        if (rc) return rc;
+        while (left >= G) {
+            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            left -= G;
+        }
+    }
+    return launch_direct(ctx, left, nullptr);
+}
+
 int check_flags(pdpd_ctx *ctx)
 {
     uint32_t f = 0;
@@ -550,10 +494,7 @@ int check_flags(pdpd_ctx *ctx)
     ctx->stats.flags |= f;
     HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (f & PDPD_FLAG_TILE_OVERFLOW)
-        return fail(PDPD_ERR_CAPACITY,
-                    "a brick's halo held more beads than the LDS tile (" + std::to_string(kTileBeads) +
-                        "); set a smaller \"brick\"");
+    if (f & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
     return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
 }
 
@@ -582,7 +523,21 @@ std::string support_reason(const pdpd_world_desc *w)
         uint32_t len = w->polymer_bead_start[p + 1] - w->polymer_bead_start[p];
         if (len > kMaxPolymerLength) return "Polymer longer than 127 beads.";
     }
+    if (w->n_beads >= kMaxLayoutBeads) return "World has more than 2^26 beads.";
     return "";
+}
+
+// Download the three state arrays of a world (current buffers).
+int fetch_world(pdpd_ctx *ctx, const WorldHost &w, std::vector<float4> &P, std::vector<float4> &V,
+                std::vector<float4> &F)
+{
+    P.resize(w.n); V.resize(w.n); F.resize(w.n);
+    if (!w.n) return PDPD_OK;
+    const int c = ctx->cur;
+    HIP_TRY(hipMemcpy(P.data(), ctx->pos[c].p + w.bead_off, w.n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(V.data(), ctx->vel[c].p + w.bead_off, w.n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(F.data(), ctx->frc[c].p + w.bead_off, w.n * sizeof(float4), hipMemcpyDeviceToHost));
+    return PDPD_OK;
 }
 
 }  // namespace
@@ -602,7 +557,7 @@ int pdpd_get_limits(pdpd_limits *out)
     out->min_box = 3;
     out->max_box = kMaxBox;
     out->max_polymer_length = kMaxPolymerLength;
-    out->tile_capacity = kTileBeads;
+    out->max_beads = kMaxLayoutBeads - 1;
     out->max_bonds_per_bead = 0xffffffffu;
     return PDPD_OK;
 }
@@ -661,27 +616,22 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
     w.up_pos.resize(w.n);
     w.up_vel.resize(w.n);
     w.up_frc.resize(w.n);
-    w.cell_occupancy.assign(w.n_cells(), 0);
     for (uint32_t i = 0; i < w.n; i++) {
         float x[3], v[3], f[3];
-        int c[3];
         for (int k = 0; k < 3; k++) {
+            if (!(d->x[3 * i + k] >= 0.0) || !(d->x[3 * i + k] < (double)w.L[k]))
+                return fail(PDPD_ERR_INVALID, "bead " + std::to_string(i) + " outside the box");
             const float Lf = (float)w.L[k];
             x[k] = (float)d->x[3 * i + k];
-            if (!(x[k] >= 0.0f)) return fail(PDPD_ERR_INVALID, "bead " + std::to_string(i) + " outside the box");
             if (x[k] >= Lf) x[k] = std::nextafter(Lf, 0.0f);   // FP32 rounding of x < L up to L
             v[k] = (float)d->v[3 * i + k];
             f[k] = (float)d->f[3 * i + k];
             v[k] = std::fmaf(f[k], -hdt, v[k]);
-            c[k] = std::min((int)std::floor(x[k]), w.L[k] - 1);
         }
-        if (d->x[3 * i] >= w.L[0] || d->x[3 * i + 1] >= w.L[1] || d->x[3 * i + 2] >= w.L[2])
-            return fail(PDPD_ERR_INVALID, "bead " + std::to_string(i) + " outside the box");
         const uint32_t h = bead_hash(d->bead_type[i], d->is_monomer[i] != 0, d->polymer_id[i], d->polymer_offset[i]);
         w.up_pos[i] = make_float4(x[0], x[1], x[2], u2f(h));
         w.up_vel[i] = make_float4(v[0], v[1], v[2], 0.0f);
         w.up_frc[i] = make_float4(f[0], f[1], f[2], 0.0f);
-        w.cell_occupancy[(size_t)(c[0] * w.L[1] + c[1]) * w.L[2] + c[2]]++;
     }
     // topology in the reference's order: per polymer, its type's bonds then bond pairs
     for (uint32_t p = 0; p < d->n_polymers; p++) {
@@ -689,13 +639,10 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
         if (pt >= d->n_polymer_types) return fail(PDPD_ERR_INVALID, "polymer type out of range");
         const uint32_t *ids = d->polymer_bead_ids + d->polymer_bead_start[p];
         const uint32_t len = d->polymer_bead_start[p + 1] - d->polymer_bead_start[p];
-        WorldHost::Poly poly{(uint32_t)w.bonds.size(), 0, 0, 0};
         for (uint32_t b = d->pt_bond_start[pt]; b < d->pt_bond_start[pt + 1]; b++) {
             if (d->bond_head[b] >= len || d->bond_tail[b] >= len) return fail(PDPD_ERR_INVALID, "bond offset out of range");
             w.bonds.push_back({ids[d->bond_head[b]], ids[d->bond_tail[b]], (float)d->bond_kappa[b], (float)d->bond_r0[b]});
         }
-        poly.b1 = (uint32_t)w.bonds.size();
-        poly.a0 = (uint32_t)w.angles.size();
         for (uint32_t a = d->pt_angle_start[pt]; a < d->pt_angle_start[pt + 1]; a++) {
             const uint32_t hb = d->pt_bond_start[pt] + d->angle_head_bond[a];
             const uint32_t tb = d->pt_bond_start[pt] + d->angle_tail_bond[a];
@@ -710,8 +657,6 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
             r.sin0 = (float)std::sin(d->angle_theta0[a]);
             w.angles.push_back(r);
         }
-        poly.a1 = (uint32_t)w.angles.size();
-        if (poly.b1 > poly.b0 || poly.a1 > poly.a0) w.polys.push_back(poly);
     }
     for (const auto &b : w.bonds)
         if (b.head >= w.n || b.tail >= w.n) return fail(PDPD_ERR_INVALID, "bond bead id out of range");
@@ -757,17 +702,7 @@ int pdpd_run(pdpd_ctx *ctx, uint32_t nsteps)
     uint64_t bs = 0;
     rc = load_counters(ctx, nsteps, &bs);
     if (rc) return rc;
-    uint32_t left = nsteps;
-    const uint32_t G = ctx->graph_steps & ~1u;     // even: buffers return to the same parity
-    if (G >= 2 && left >= G) {
-        rc = ensure_graph(ctx, G);
-        if (rc) return rc;
-        while (left >= G) {
-            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-            left -= G;
-        }
-    }
-    rc = launch_direct(ctx, left, nullptr);
+    rc = enqueue_steps(ctx, nsteps);
     if (rc) return rc;
     finish_counters(ctx, nsteps);
     ctx->stats.steps += nsteps;
@@ -795,13 +730,9 @@ int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f
     if (rc) return rc;
     rc = pdpd_sync(ctx);
     if (rc) return rc;
-    std::vector<float4> P(w->n), V(w->n), F(w->n);
-    const int c = ctx->cur;
-    if (w->n) {
-        HIP_TRY(hipMemcpy(P.data(), ctx->pos[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(V.data(), ctx->vel[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(F.data(), ctx->frc[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-    }
+    std::vector<float4> P, V, F;
+    rc = fetch_world(ctx, *w, P, V, F);
+    if (rc) return rc;
     const double hdt = (double)(float)w->dt * 0.5;
     for (uint32_t s = 0; s < w->n; s++) {
         const uint32_t i = f2u(V[s].w) - w->bead_off;
@@ -827,13 +758,9 @@ int pdpd_world_download_f32(pdpd_ctx *ctx, int slot, float *x, float *wout, floa
     if (rc) return rc;
     rc = pdpd_sync(ctx);
     if (rc) return rc;
-    std::vector<float4> P(w->n), V(w->n), F(w->n);
-    const int c = ctx->cur;
-    if (w->n) {
-        HIP_TRY(hipMemcpy(P.data(), ctx->pos[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(V.data(), ctx->vel[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(F.data(), ctx->frc[c].p + w->bead_off, w->n * sizeof(float4), hipMemcpyDeviceToHost));
-    }
+    std::vector<float4> P, V, F;
+    rc = fetch_world(ctx, *w, P, V, F);
+    if (rc) return rc;
     for (uint32_t s = 0; s < w->n; s++) {
         const uint32_t i = f2u(V[s].w) - w->bead_off;
         if (i >= w->n) return fail(PDPD_ERR_STATE, "corrupt bead index in device state");
@@ -868,34 +795,29 @@ int pdpd_world_download_cells(pdpd_ctx *ctx, int slot, int32_t *order, int32_t *
     return PDPD_OK;
 }
 
+static int parse_u32(const std::string &v, uint32_t lo, uint32_t hi, uint32_t *out)
+{
+    char *end = nullptr;
+    long long n = strtoll(v.c_str(), &end, 10);
+    if (v.empty() || !end || *end || n < (long long)lo || n > (long long)hi) return -1;
+    *out = (uint32_t)n;
+    return 0;
+}
+
 int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
 {
     if (!ctx || !key || !value) return fail(PDPD_ERR_INVALID, "null argument");
     std::string k(key), v(value);
-    if (k == "brick") {
-        int B[3];
-        if (v != "auto" && !parse_brick(v, B))
-            return fail(PDPD_ERR_INVALID, "brick must be \"auto\" or BXxBYxBZ with (BX+2)(BY+2)(BZ+2) <= " +
-                                              std::to_string(kMaxHaloCells));
-        ctx->brick_cfg = v;
-        if (!ctx->dirty && ctx->n_worlds) {
-            choose_bricks(ctx);
-            return build_brick_table(ctx);
-        }
-        return PDPD_OK;
-    }
     if (k == "graph_steps") {
-        char *end = nullptr;
-        long n = strtol(v.c_str(), &end, 10);
-        if (!end || *end || n < 0 || n > 100000) return fail(PDPD_ERR_INVALID, "graph_steps must be an integer >= 0");
-        ctx->graph_steps = (uint32_t)n;
+        uint32_t n;
+        if (parse_u32(v, 0, 100000, &n)) return fail(PDPD_ERR_INVALID, "graph_steps must be an integer in [0, 100000]");
+        ctx->graph_steps = n;
         return PDPD_OK;
     }
-    if (k == "persistent_blocks") {
-        char *end = nullptr;
-        long n = strtol(v.c_str(), &end, 10);
-        if (!end || *end || n < 0 || n > (1 << 24)) return fail(PDPD_ERR_INVALID, "persistent_blocks must be >= 0");
-        ctx->persistent_blocks = (uint32_t)n;
+    if (k == "force_grid") {
+        uint32_t n;
+        if (parse_u32(v, 0, 1u << 24, &n)) return fail(PDPD_ERR_INVALID, "force_grid must be an integer >= 0");
+        ctx->force_grid = n;
         ctx->graph_version = ~0ull;
         return PDPD_OK;
     }
@@ -906,9 +828,8 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
 {
     if (!ctx || !key || !buf || !buflen) return fail(PDPD_ERR_INVALID, "null argument");
     std::string k(key), v;
-    if (k == "brick") v = ctx->brick_cfg;
-    else if (k == "graph_steps") v = std::to_string(ctx->graph_steps);
-    else if (k == "persistent_blocks") v = std::to_string(ctx->persistent_blocks);
+    if (k == "graph_steps") v = std::to_string(ctx->graph_steps);
+    else if (k == "force_grid") v = std::to_string(ctx->force_grid);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -918,10 +839,9 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
 {
     if (!ctx || !buf || !buflen) return fail(PDPD_ERR_INVALID, "null argument");
     std::string s;
-    s += "brick=" + ctx->brick_cfg + "\tcells per pair-force workgroup, \"auto\" or BXxBYxBZ\n";
     s += "graph_steps=" + std::to_string(ctx->graph_steps) + "\tsteps per captured hipGraph (even; 0 = direct launches)\n";
-    s += "persistent_blocks=" + std::to_string(ctx->persistent_blocks) +
-         "\tpair-force grid size for a persistent launch (0 = one workgroup per brick)\n";
+    s += "force_grid=" + std::to_string(ctx->force_grid) +
+         "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }
@@ -970,6 +890,8 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps)
     return check_flags(ctx);
 }
 
+// Times launch shapes that do not change results (the pair-force grid and the
+// graph length) on a saved copy of the state, keeps the fastest.
 int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len)
 {
     if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
@@ -977,10 +899,8 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     int rc = ensure_layout(ctx);
     if (rc) return rc;
     if (ctx->n_worlds == 0) return fail(PDPD_ERR_INVALID, "no worlds to tune on");
-    if (steps_per_trial < 2) steps_per_trial = 2;
-    steps_per_trial &= ~1u;
+    steps_per_trial = std::max<uint32_t>(4, steps_per_trial) & ~1u;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    // save the state: trials run on a copy and the worlds do not advance
     const size_t nb = ctx->n_beads;
     DevBuf<float4> sp, sv, sf;
     HIP_TRY(dev_alloc(sp, nb));
@@ -990,90 +910,58 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     HIP_TRY(hipMemcpy(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
     HIP_TRY(hipMemcpy(sv.p, ctx->vel[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
     HIP_TRY(hipMemcpy(sf.p, ctx->frc[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    std::vector<int64_t> saved_t;
-    for (int s : ctx->order) saved_t.push_back(ctx->worlds.at(s).t);
-
-    const std::string saved_cfg = ctx->brick_cfg;
-    const uint32_t saved_persist = ctx->persistent_blocks;
-    std::string best;
-    uint32_t best_persist = 0;
+    auto restore = [&]() -> int {
+        HIP_TRY(hipMemcpy(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
+        ctx->cur = c0;
+        return PDPD_OK;
+    };
+    int cus = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    const uint32_t grids[] = {0u, (uint32_t)cus * 4, (uint32_t)cus * 8, (uint32_t)cus * 16};
+    const uint32_t graphs[] = {10u, 20u, 50u};
+    const uint32_t saved_grid = ctx->force_grid, saved_graph = ctx->graph_steps;
+    uint32_t best_grid = saved_grid, best_graph = saved_graph;
     double best_ms = 1e30;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const uint32_t persist_opts[2] = {0, 256u * 8u};
-    for (const auto &cand : kBrickCandidates) {
-        if (!brick_fits_lds(cand)) continue;
-        bool ok = true;
-        for (auto &kv : ctx->worlds) {
-            uint32_t h = max_halo_count(kv.second, cand);
-            if (h * 5 / 4 + 16 > (uint32_t)kTileBeads) ok = false;
-        }
-        if (!ok) continue;
-        char name[32];
-        snprintf(name, sizeof(name), "%dx%dx%d", cand[0], cand[1], cand[2]);
-        for (uint32_t po : persist_opts) {
-            ctx->brick_cfg = name;
-            ctx->persistent_blocks = po;
-            choose_bricks(ctx);
-            rc = build_brick_table(ctx);
-            if (rc) return rc;
-            for (int s = 0; s < 2; s++) {   // restore state for each trial
-                HIP_TRY(hipMemcpy(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-                HIP_TRY(hipMemcpy(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-                HIP_TRY(hipMemcpy(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-                ctx->cur = c0;
-                rc = load_counters(ctx, steps_per_trial, nullptr);
-                if (rc) return rc;
-                if (s == 0) {     // warm-up
-                    rc = launch_direct(ctx, 2, nullptr);
-                    if (rc) return rc;
-                    HIP_TRY(hipStreamSynchronize(ctx->stream));
-                    continue;
-                }
-                HIP_TRY(hipEventRecord(e0, ctx->stream));
-                rc = launch_direct(ctx, steps_per_trial, nullptr);
-                if (rc) return rc;
-                HIP_TRY(hipEventRecord(e1, ctx->stream));
-                HIP_TRY(hipEventSynchronize(e1));
+    for (uint32_t g : grids) {
+        for (uint32_t gs : graphs) {
+            ctx->force_grid = g;
+            ctx->graph_steps = gs;
+            ctx->graph_version = ~0ull;
+            const uint32_t steps = std::max(steps_per_trial, gs);
+            for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
+                if ((rc = restore())) return rc;
+                if ((rc = load_counters(ctx, steps, nullptr))) return rc;
+                if (pass) HIP_TRY(hipEventRecord(e0, ctx->stream));
+                if ((rc = enqueue_steps(ctx, steps))) return rc;
+                if (pass) HIP_TRY(hipEventRecord(e1, ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
             }
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            uint32_t f = 0;
-            HIP_TRY(hipMemcpy(&f, ctx->flags.p, sizeof(f), hipMemcpyDeviceToHost));
-            if (f) {      // candidate overflowed: discard it
-                HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
-                continue;
-            }
-            if (ms < best_ms) { best_ms = ms; best = name; best_persist = po; }
+            const double per = ms / steps;
+            if (per < best_ms) { best_ms = per; best_grid = g; best_graph = gs; }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // restore
-    HIP_TRY(hipMemcpy(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    ctx->cur = c0;
-    for (size_t i = 0; i < ctx->order.size(); i++) ctx->worlds.at(ctx->order[i]).t = saved_t[i];
+    if ((rc = restore())) return rc;
     sp.release(); sv.release(); sf.release();
-    if (best.empty()) {
-        ctx->brick_cfg = saved_cfg;
-        ctx->persistent_blocks = saved_persist;
-        choose_bricks(ctx);
-        rc = build_brick_table(ctx);
-        if (rc) return rc;
-        return fail(PDPD_ERR_CAPACITY, "no brick shape fits the worlds");
-    }
-    ctx->brick_cfg = best;
-    ctx->persistent_blocks = best_persist;
-    choose_bricks(ctx);
-    rc = build_brick_table(ctx);
-    if (rc) return rc;
+    ctx->force_grid = best_grid;
+    ctx->graph_steps = best_graph;
+    ctx->graph_version = ~0ull;
+    (void)saved_grid; (void)saved_graph;
+    if ((rc = check_flags(ctx))) return rc;
     if (choice && choice_len)
-        snprintf(choice, choice_len, "brick=%s persistent_blocks=%u ms_per_step=%.5f", best.c_str(), best_persist,
-                 best_ms / steps_per_trial);
+        snprintf(choice, choice_len, "force_grid=%u graph_steps=%u ms_per_step=%.5f", best_grid, best_graph, best_ms);
     return PDPD_OK;
 }
 

@@ -8,15 +8,15 @@
 namespace pdpd {
 
 constexpr int kMaxTypes = 16;          // 4-bit type field of the bead hash
-constexpr int kForceThreads = 256;     // pair-force workgroup (4 wave64)
-constexpr int kMaxHaloCells = 512;     // (Bx+2)(By+2)(Bz+2) of one brick
-constexpr int kMaxInnerCells = 256;    // Bx*By*Bz of one brick
-constexpr int kTileBeads = 1280;       // beads of one brick's halo staged in LDS (u16 indices)
-constexpr int kPairList = 16;          // accepted-pair list per lane in the pair-force kernel
+constexpr int kBeadThreads = 256;      // bead-parallel kernels
+constexpr int kForceBeads = 64;        // beads per pair-force workgroup (one per lane of a wave)
+constexpr int kForcePlanes = 3;        // waves per pair-force workgroup: one per x-plane of the stencil
+constexpr int kForceThreads = kForceBeads * kForcePlanes;
+constexpr int kPairList = 16;          // accepted candidates buffered per lane before evaluation
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 8;
-constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan block
-constexpr int kBeadThreads = 256;
+constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan tile
+constexpr uint32_t kMaxLayoutBeads = 1u << 26;              // pair-list entries carry 26-bit slots
 
 // Per-world parameters, read by every kernel through the world index.
 struct WorldDev {
@@ -28,10 +28,9 @@ struct WorldDev {
     uint32_t n_beads;
     uint32_t cell_off;     // first global cell id
     uint32_t n_cells;
-    int32_t brick[3];      // brick shape of the pair-force work items
     uint32_t seed_lo, seed_hi;
     uint32_t n_types;
-    uint32_t pad0;
+    uint32_t pad0[3];
     float a[kMaxTypes * kMaxTypes];    // conservative a_ij
     float sg[kMaxTypes * kMaxTypes];   // sqrt(gamma_ij)
 };
@@ -42,12 +41,6 @@ struct WorldCtr {
     uint32_t remaining;    // steps this world may still advance in the current run
 };
 
-// One pair-force work item: a brick of cells of one world.
-struct BrickItem {
-    int32_t world;
-    int32_t bx, by, bz;    // brick origin in cells
-};
-
 // One bead-parallel block: [begin, end) slots of one world.
 struct BeadBlock {
     int32_t world;
@@ -55,16 +48,17 @@ struct BeadBlock {
     uint32_t pad;
 };
 
-// Bonded topology, flattened per polymer in the reference's order
-// (bonds of the polymer type, then its bond pairs).
-struct PolyItem {
-    int32_t world;
-    uint32_t bond_begin, bond_end;
-    uint32_t angle_begin, angle_end;
-    uint32_t pad[3];
-};
-struct BondRec { uint32_t head, tail; float kappa, r0; };               // global bead indices
+// Bonded topology. Bond and angle records hold global bead indices, in the
+// reference's order (per polymer: the type's bonds, then its bond pairs).
+struct BondRec { uint32_t head, tail; float kappa, r0; };
 struct AngleRec { uint32_t head, mid, tail, pad; float kappa, cos0, sin0, pad1; };
+
+// A bead that takes part in bonds: its terms are bead_terms[t0, t1), each
+// (role << 29) | record index, in the order the per-polymer loop applies them.
+struct BondedBead { uint32_t bead; uint32_t t0, t1; int32_t world; };
+constexpr uint32_t kRoleShift = 29;
+constexpr uint32_t kRoleBondHead = 0, kRoleBondTail = 1, kRoleAngleHead = 2, kRoleAngleTail = 3,
+                   kRoleAngleMid = 4;
 
 // Device buffers of one layout.
 struct DeviceState {
@@ -74,16 +68,19 @@ struct DeviceState {
     uint32_t *key;         // cell id per slot after the drift
     uint32_t *rank;        // arrival rank inside the cell | inactive bit
     uint32_t *perm;        // sorted position -> old slot | inactive bit
+    uint32_t *sorted_idx;  // sorted position -> global bead index
     uint32_t *cell_count;  // n_cells_total
     uint32_t *cell_start;  // n_cells_total + 1
-    uint32_t *block_sums;  // scan partials
+    unsigned long long *scan_tiles;   // look-back state per scan tile
+    uint32_t *scan_ticket;
     uint32_t *slot_of_bead;
     uint32_t *flags;
     WorldDev *worlds;
     WorldCtr *ctr;
-    BrickItem *bricks;
-    BeadBlock *bead_blocks;
-    PolyItem *polys;
+    BeadBlock *bead_blocks;    // kBeadThreads beads per block
+    BeadBlock *force_blocks;   // kForceBeads beads per block
+    BondedBead *bonded;
+    uint32_t *bead_terms;
     BondRec *bonds;
     AngleRec *angles;
 };

@@ -13,9 +13,9 @@ struct StepLaunch {
     uint32_t n_cells;
     uint32_t n_worlds;
     uint32_t n_bead_blocks;
-    uint32_t n_bricks;
-    uint32_t n_polys;
-    uint32_t force_blocks;   // 0 = one workgroup per brick, else persistent grid size
+    uint32_t n_force_blocks;
+    uint32_t n_bonded;
+    uint32_t force_grid;     // 0 = one workgroup per force block, else persistent grid size
 };
 
 // Events recorded between the kernels of one step (profiling runs only):
@@ -24,6 +24,7 @@ struct KernelEvents {
     hipEvent_t ev[7];
 };
 
+uint32_t scan_tiles(uint32_t n_cells);
 hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);

@@ -4,16 +4,20 @@
 //   k_integrate   finish-kick + drift + half-kick + wrap, cell key, arrival rank
 //                 (dpd_maths_core_half_step_raw.hpp:42-72,114-124 fused as
 //                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does)
-//   k_scan_*      exclusive prefix sum of cell counts -> cell starts (counting sort)
+//   k_scan        single-pass exclusive scan of the cell counts (look-back)
 //   k_scatter     sorted position of every bead's slot
 //   k_reorder     stable gather into the other buffer, bead-index order in a cell
-//   k_force       pair forces: a brick of cells + its halo staged in LDS, one lane
-//                 per home bead over the 27-cell stencil, register accumulation,
-//                 no atomics (dpd_maths_core_half_step_raw.hpp:129-216; the RNG of
+//   k_force       pair forces: per workgroup 64 consecutive beads x 3 waves, one
+//                 wave per x-plane of the 27-cell stencil; neighbours are read
+//                 straight from the cell-sorted arrays (contiguous z-rows); each
+//                 lane buffers the candidates inside the cutoff and then
+//                 evaluates them; register accumulation, no atomics
+//                 (dpd_maths_core_half_step_raw.hpp:129-216; RNG of
 //                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79)
-//   k_bonds       Hookean + angle bonds per polymer, then the per-world step
-//                 counter (naive_dpd_engine_half_merge.hpp:638-690,
-//                 dpd_maths_core_half_step.hpp:178-231)
+//   k_bonds       Hookean + angle terms per bonded bead, in the per-polymer
+//                 order of naive_dpd_engine_half_merge.hpp:376-385 (terms of
+//                 :638-690 / dpd_maths_core_half_step.hpp:178-231), then the
+//                 per-world step counter
 // The final half-kick v += f dt/2 of a step is folded into the next k_integrate
 // (the engine stores w = v - f dt/2).
 //
@@ -31,6 +35,9 @@ namespace pdpd {
 constexpr float kMinR2 = 1e-9f * 1e-9f;           // MIN_DISTANCE_CUTOFF_SQR
 constexpr float kTwoM32 = 0.00000000023283064365386962890625f;  // 2^-32
 constexpr uint32_t kInactive = 0x80000000u;
+constexpr uint32_t kSlotMask = kMaxLayoutBeads - 1;
+constexpr unsigned long long kTileAgg = 1ull << 32, kTileIncl = 2ull << 32;
+constexpr uint32_t kScanSpinLimit = 1u << 24;
 
 // ---------------------------------------------------------------- RNG ----
 
@@ -113,10 +120,22 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *s_wsum,
     return base + x - v;
 }
 
+__device__ __forceinline__ int cell_coord(float x, int L)
+{
+    int c = (int)floorf(x);
+    c = c < 0 ? 0 : c;
+    return c >= L ? L - 1 : c;
+}
+
 // ---------------------------------------------------------- integrate ----
 
-__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur)
+__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur, uint32_t n_tiles, uint32_t n_blocks)
 {
+    if (blockIdx.x == 0) {       // reset the scan's look-back state for this step
+        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x) S.scan_tiles[i] = 0ull;
+        if (threadIdx.x == 0) *S.scan_ticket = 0u;
+    }
+    if (blockIdx.x >= n_blocks) return;
     const BeadBlock blk = S.bead_blocks[blockIdx.x];
     const uint32_t s = blk.begin + threadIdx.x;
     if (s >= blk.end) return;
@@ -145,19 +164,12 @@ __global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int c
         S.pos[cur][s] = make_float4(x[0], x[1], x[2], p.w);
         S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
     }
-    int c[3];
     bool bad = false;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const float Lf = W->Lf[d];
-        bad |= !(x[d] >= 0.0f && x[d] < Lf);
-        int cc = (int)floorf(x[d]);
-        cc = cc < 0 ? 0 : cc;
-        cc = cc >= W->L[d] ? W->L[d] - 1 : cc;
-        c[d] = cc;
-    }
+    for (int d = 0; d < 3; d++) bad |= !(x[d] >= 0.0f && x[d] < W->Lf[d]);
     if (bad) atomicOr(S.flags, PDPD_FLAG_BAD_POSITION);
-    const uint32_t key = W->cell_off + (uint32_t)((c[0] * W->L[1] + c[1]) * W->L[2] + c[2]);
+    const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
+    const uint32_t key = W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2);
     const uint32_t r = atomicAdd(&S.cell_count[key], 1u);
     S.key[s] = key;
     S.rank[s] = r | (active ? 0u : kInactive);
@@ -165,59 +177,65 @@ __global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int c
 
 // --------------------------------------------------------------- scan ----
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ count, uint32_t n,
-                                                             uint32_t *__restrict__ block_sums)
+// Single-pass exclusive scan of the cell counts with decoupled look-back: tiles
+// are taken in ticket order, so every tile a block waits on belongs to a block
+// that has already started and publishes its aggregate without waiting. The
+// spin is bounded; running out raises PDPD_FLAG_SCAN_TIMEOUT.
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ count, uint32_t n,
+                                                      uint32_t *__restrict__ cell_start,
+                                                      unsigned long long *__restrict__ tiles,
+                                                      uint32_t *__restrict__ ticket, uint32_t n_tiles,
+                                                      uint32_t *__restrict__ flags)
 {
     __shared__ uint32_t s_wsum[4];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
-    uint32_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPerThread; k++) {
-        uint32_t i = base + k;
-        acc += (i < n) ? count[i] : 0u;
-    }
-    uint32_t tot;
-    block_scan_256(acc, s_wsum, &tot);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
-}
-
-// One block: exclusive scan of the block sums in place; writes the grand total
-// to cell_start[n_cells].
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t *__restrict__ block_sums, uint32_t nb,
-                                                          uint32_t *__restrict__ cell_start, uint32_t n_cells)
-{
-    __shared__ uint32_t s_wsum[4];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += kScanThreads) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = (i < nb) ? block_sums[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_scan_256(v, s_wsum, &tot);
-        if (i < nb) block_sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) cell_start[n_cells] = carry;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict__ count, uint32_t n,
-                                                           const uint32_t *__restrict__ block_sums,
-                                                           uint32_t *__restrict__ cell_start)
-{
-    __shared__ uint32_t s_wsum[4];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
+    __shared__ uint32_t s_tile, s_prefix;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kScanBlock + threadIdx.x * kScanPerThread;
     uint32_t v[kScanPerThread];
     uint32_t acc = 0;
 #pragma unroll
     for (int k = 0; k < kScanPerThread; k++) {
-        uint32_t i = base + k;
+        const uint32_t i = base + k;
         v[k] = (i < n) ? count[i] : 0u;
         acc += v[k];
     }
-    uint32_t tot;
-    uint32_t ex = block_scan_256(acc, s_wsum, &tot) + block_sums[blockIdx.x];
+    uint32_t agg;
+    uint32_t ex = block_scan_256(acc, s_wsum, &agg);
+    if (threadIdx.x == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&tiles[0], kTileIncl | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&tiles[tile], kTileAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int j = (int)tile - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const unsigned long long st = __hip_atomic_load(&tiles[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long status = st & 0xffffffff00000000ull;
+                if (status == 0ull) {
+                    if (++spins > kScanSpinLimit) {
+                        atomicOr(flags, PDPD_FLAG_SCAN_TIMEOUT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += (uint32_t)st;
+                if (status == kTileIncl) break;
+                j--;
+            }
+            __hip_atomic_store(&tiles[tile], kTileIncl | (prefix + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+        if (tile == n_tiles - 1) cell_start[n] = prefix + agg;
+    }
+    __syncthreads();
+    ex += s_prefix;
 #pragma unroll
     for (int k = 0; k < kScanPerThread; k++) {
-        uint32_t i = base + k;
+        const uint32_t i = base + k;
         if (i < n) {
             cell_start[i] = ex;
             count[i] = 0u;          // ready for the next step's arrivals
@@ -228,12 +246,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
 
 // ------------------------------------------------------- sort scatter ----
 
-__global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, uint32_t n)
+__global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur, uint32_t n)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t r = S.rank[s];
-    S.perm[S.cell_start[S.key[s]] + (r & ~kInactive)] = s | (r & kInactive);
+    const uint32_t j = S.cell_start[S.key[s]] + (r & ~kInactive);
+    S.perm[j] = s | (r & kInactive);
+    S.sorted_idx[j] = fbits(S.vel[cur][s].w);
 }
 
 // Stable order inside a cell: beads ranked by global bead index, so the order
@@ -247,153 +267,66 @@ __global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur
     const uint32_t s = e & ~kInactive;
     const uint32_t k = S.key[s];
     const uint32_t a = S.cell_start[k], b = S.cell_start[k + 1];
-    const float4 v = S.vel[cur][s];
-    const uint32_t me = fbits(v.w);
+    const uint32_t me = S.sorted_idx[j];
     uint32_t rank = 0;
-    for (uint32_t q = a; q < b; q++) {
-        const uint32_t s2 = S.perm[q] & ~kInactive;
-        rank += fbits(S.vel[cur][s2].w) < me ? 1u : 0u;
-    }
+    for (uint32_t q = a; q < b; q++) rank += S.sorted_idx[q] < me ? 1u : 0u;
     const uint32_t dst = a + rank;
     S.pos[nxt][dst] = S.pos[cur][s];
-    S.vel[nxt][dst] = v;
+    S.vel[nxt][dst] = S.vel[cur][s];
     if (e & kInactive) S.frc[nxt][dst] = S.frc[cur][s];
     S.slot_of_bead[me] = dst;
 }
 
 // --------------------------------------------------------- pair force ----
 
-__global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt, uint32_t n_items)
+// Shift codes of a candidate's periodic image, 2 bits per axis: 0 none,
+// 1 = +L, 2 = -L. Stored with the candidate's slot in the per-lane list.
+__device__ __forceinline__ float shift_of(uint32_t code, float L)
 {
-    __shared__ float4 s_pos[kTileBeads];
-    __shared__ float4 s_vel[kTileBeads];
-    __shared__ uint32_t s_cstart[kMaxHaloCells + 1];
-    __shared__ uint32_t s_cglob[kMaxHaloCells];
-    __shared__ uint32_t s_hstart[kMaxInnerCells + 1];
-    __shared__ float s_a[kMaxTypes * kMaxTypes];
-    __shared__ float s_sg[kMaxTypes * kMaxTypes];
-    __shared__ uint32_t s_wsum[4];
-    __shared__ uint16_t s_list[kPairList * kForceThreads];
+    return code == 1u ? L : (code == 2u ? -L : 0.0f);
+}
+
+__global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
+{
+    __shared__ uint32_t s_list[kPairList * kForceThreads];
+    __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
 
     const int tid = threadIdx.x;
-    const WorldCtr *__restrict__ ctr = S.ctr;   // this step's counters
-    int cur_world = -1;
+    const int plane = tid / kForceBeads;            // wave-uniform
+    const int lane = tid % kForceBeads;
+    const float4 *__restrict__ pos = S.pos[nxt];
+    const float4 *__restrict__ vel = S.vel[nxt];
+    const uint32_t *__restrict__ cs = S.cell_start;
 
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const BrickItem it = S.bricks[item];
-        if (ctr[it.world].remaining == 0) continue;
-        const WorldDev *__restrict__ W = &S.worlds[it.world];
-        const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
-        const int n0 = min(W->brick[0], L0 - it.bx);
-        const int n1 = min(W->brick[1], L1 - it.by);
-        const int n2 = min(W->brick[2], L2 - it.bz);
-        const int h1 = n1 + 2, h2 = n2 + 2;
-        const int nh = (n0 + 2) * h1 * h2;
-        const int ni = n0 * n1 * n2;
-
-        __syncthreads();    // previous item is done with LDS
-        if (it.world != cur_world) {
-            s_a[tid] = W->a[tid];
-            s_sg[tid] = W->sg[tid];
-            cur_world = it.world;
-        }
-
-        // 1. halo cell table: global start and count of every halo cell
-        uint32_t cnt[2];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int hc = tid * 2 + q;
-            cnt[q] = 0;
-            if (hc < nh) {
-                const int iz = hc % h2, iy = (hc / h2) % h1, ix = hc / (h1 * h2);
-                int gx = it.bx - 1 + ix, gy = it.by - 1 + iy, gz = it.bz - 1 + iz;
-                gx += gx < 0 ? L0 : 0; gx -= gx >= L0 ? L0 : 0;
-                gy += gy < 0 ? L1 : 0; gy -= gy >= L1 ? L1 : 0;
-                gz += gz < 0 ? L2 : 0; gz -= gz >= L2 ? L2 : 0;
-                const uint32_t c = W->cell_off + (uint32_t)((gx * L1 + gy) * L2 + gz);
-                const uint32_t st = S.cell_start[c];
-                s_cglob[hc] = st;
-                cnt[q] = S.cell_start[c + 1] - st;
-            }
-        }
-        uint32_t total;
-        const uint32_t ex = block_scan_256(cnt[0] + cnt[1], s_wsum, &total);
-        if (tid * 2 < nh) s_cstart[tid * 2] = ex;
-        if (tid * 2 + 1 < nh) s_cstart[tid * 2 + 1] = ex + cnt[0];
-        if (tid == 0) s_cstart[nh] = total;
-        __syncthreads();
-
-        // inner (home) cells: prefix of their counts
-        uint32_t hcnt = 0;
-        int my_hc = 0;
-        if (tid < ni) {
-            const int jz = tid % n2, jy = (tid / n2) % n1, jx = tid / (n1 * n2);
-            my_hc = ((jx + 1) * h1 + (jy + 1)) * h2 + (jz + 1);
-            hcnt = s_cstart[my_hc + 1] - s_cstart[my_hc];
-        }
-        uint32_t n_home;
-        const uint32_t hex = block_scan_256(hcnt, s_wsum, &n_home);
-        if (tid < ni) s_hstart[tid] = hex;
-        if (tid == 0) s_hstart[ni] = n_home;
-
-        if (total > (uint32_t)kTileBeads) {           // uniform: skip, host reports
-            if (tid == 0) atomicOr(S.flags, PDPD_FLAG_TILE_OVERFLOW);
-            continue;
-        }
-
-        // 2. stage halo beads (positions shifted by the periodic image)
-        for (uint32_t j = tid; j < total; j += kForceThreads) {
-            int lo = 0, hi = nh;                       // last hc with s_cstart[hc] <= j
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_cstart[mid] <= j) lo = mid; else hi = mid;
-            }
-            const int hc = lo;
-            const uint32_t g = s_cglob[hc] + (j - s_cstart[hc]);
-            const int iz = hc % h2, iy = (hc / h2) % h1, ix = hc / (h1 * h2);
-            const int gx = it.bx - 1 + ix, gy = it.by - 1 + iy, gz = it.bz - 1 + iz;
-            const float sx = gx < 0 ? -W->Lf[0] : (gx >= L0 ? W->Lf[0] : 0.0f);
-            const float sy = gy < 0 ? -W->Lf[1] : (gy >= L1 ? W->Lf[1] : 0.0f);
-            const float sz = gz < 0 ? -W->Lf[2] : (gz >= L2 ? W->Lf[2] : 0.0f);
-            float4 p = S.pos[nxt][g];
-            p.x += sx; p.y += sy; p.z += sz;
-            s_pos[j] = p;
-            s_vel[j] = S.vel[nxt][g];
-        }
-        __syncthreads();
-
-        // 3. one lane per home bead, 27 cells as 9 contiguous z-rows of 3 cells.
-        //    Candidates within the cutoff are first collected into a per-lane
-        //    list in LDS, then evaluated: the expensive pair body runs once per
-        //    accepted pair instead of once per candidate of any lane in the wave.
-        //    The list keeps candidate order, so the sum order is unchanged.
-        const WorldCtr wc = ctr[it.world];
-        const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
-        const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
-        const float scale = W->rng_scale;
-        for (uint32_t k = tid; k < n_home; k += kForceThreads) {
-            int lo = 0, hi = ni;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_hstart[mid] <= k) lo = mid; else hi = mid;
-            }
-            const int ic = lo;
-            const int jz = ic % n2, jy = (ic / n2) % n1, jx = ic / (n1 * n2);
-            const int hc = ((jx + 1) * h1 + (jy + 1)) * h2 + (jz + 1);
-            const uint32_t off = k - s_hstart[ic];
-            const uint32_t li = s_cstart[hc] + off;
-            const float4 hp = s_pos[li];
-            const float4 hv = s_vel[li];
+    for (uint32_t bi = blockIdx.x; bi < n_blocks; bi += gridDim.x) {
+        const BeadBlock blk = S.force_blocks[bi];
+        if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world
+        const WorldDev *__restrict__ W = &S.worlds[blk.world];
+        const uint32_t s = blk.begin + lane;
+        const bool live = s < blk.end;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (live) {
+            const WorldCtr wc = S.ctr[blk.world];
+            const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
+            const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
+            const float scale = W->rng_scale;
+            const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
+            const float4 hp = pos[s];
+            const float4 hv = vel[s];
             const uint32_t hh = fbits(hp.w);
             const uint32_t trow = (hh >> 28) * kMaxTypes;
-            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+            const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
             uint32_t cnt = 0;
             auto flush = [&]() {
                 for (uint32_t q = 0; q < cnt; q++) {
-                    const uint32_t m = s_list[q * kForceThreads + tid];
-                    const float4 op = s_pos[m];
-                    const float4 ov = s_vel[m];
-                    const float dx = hp.x - op.x, dy = hp.y - op.y, dz = hp.z - op.z;
+                    const uint32_t e = s_list[q * kForceThreads + tid];
+                    const uint32_t m = e & kSlotMask;
+                    const float4 op = pos[m];
+                    const float4 ov = vel[m];
+                    const float ox = op.x + shift_of((e >> 26) & 3u, W->Lf[0]);
+                    const float oy = op.y + shift_of((e >> 28) & 3u, W->Lf[1]);
+                    const float oz = op.z + shift_of((e >> 30) & 3u, W->Lf[2]);
+                    const float dx = hp.x - ox, dy = hp.y - oy, dz = hp.z - oz;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     const float rr = sqrtf(r2);
                     const float inv_r = 1.0f / rr;
@@ -403,33 +336,61 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
                     const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
                     const uint32_t oh = fbits(op.w);
                     const uint32_t ti = trow + (oh >> 28);
-                    const float sgw = s_sg[ti] * wr;
+                    const float sgw = W->sg[ti] * wr;
                     const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
-                    const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                    const float F = fmaf(W->a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
                     fx = fmaf(ex, F, fx);
                     fy = fmaf(ey, F, fy);
                     fz = fmaf(ez, F, fz);
                 }
                 cnt = 0;
             };
-#pragma unroll 1
-            for (int r = 0; r < 9; r++) {
-                const int ddx = r / 3 - 1, ddy = r % 3 - 1;
-                const int rb = hc + ddx * h1 * h2 + ddy * h2;
-                const uint32_t m0 = s_cstart[rb - 1], m1 = s_cstart[rb + 2];
+            // candidates of cells [m0, m1) with one periodic image
+            auto scan = [&](uint32_t m0, uint32_t m1, uint32_t code, float sx, float sy, float sz) {
                 for (uint32_t m = m0; m < m1; m++) {
-                    const float4 op = s_pos[m];
-                    const float dx = hp.x - op.x, dy = hp.y - op.y, dz = hp.z - op.z;
+                    const float4 op = pos[m];
+                    const float dx = hp.x - (op.x + sx), dy = hp.y - (op.y + sy), dz = hp.z - (op.z + sz);
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    if (r2 < 1.0f && r2 >= kMinR2 && m != li) {
-                        s_list[cnt * kForceThreads + tid] = (uint16_t)m;
+                    if (r2 < 1.0f && r2 >= kMinR2 && m != s) {
+                        s_list[cnt * kForceThreads + tid] = m | code;
                         if (++cnt == kPairList) flush();
                     }
                 }
+            };
+            int ox = cx + plane - 1;
+            uint32_t codex = 0;
+            if (ox < 0) { ox += L0; codex = 2u; } else if (ox >= L0) { ox -= L0; codex = 1u; }
+            const float sx = shift_of(codex, W->Lf[0]);
+#pragma unroll 1
+            for (int dyi = -1; dyi <= 1; dyi++) {
+                int oy = cy + dyi;
+                uint32_t codey = 0;
+                if (oy < 0) { oy += L1; codey = 2u; } else if (oy >= L1) { oy -= L1; codey = 1u; }
+                const float sy = shift_of(codey, W->Lf[1]);
+                const uint32_t row = W->cell_off + (uint32_t)((ox * L1 + oy) * L2);
+                const uint32_t cxy = (codex << 26) | (codey << 28);
+                // z order -1, 0, +1: an image below 0 first, the contiguous run, an image above L-1 last
+                if (cz == 0) scan(cs[row + L2 - 1], cs[row + L2], cxy | (2u << 30), sx, sy, -W->Lf[2]);
+                const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz < L2 - 1 ? cz + 1 : L2 - 1;
+                scan(cs[row + zlo], cs[row + zhi + 1], cxy, sx, sy, 0.0f);
+                if (cz == L2 - 1) scan(cs[row], cs[row + 1], cxy | (1u << 30), sx, sy, W->Lf[2]);
             }
             flush();
-            S.frc[nxt][s_cglob[hc] + off] = make_float4(fx, fy, fz, 0.0f);
         }
+        // combine the three planes in order: (p0 + p1) + p2
+        if (plane > 0) {
+            s_part[plane - 1][0][lane] = fx;
+            s_part[plane - 1][1][lane] = fy;
+            s_part[plane - 1][2][lane] = fz;
+        }
+        __syncthreads();
+        if (plane == 0 && live) {
+            fx = (fx + s_part[0][0][lane]) + s_part[1][0][lane];
+            fy = (fy + s_part[0][1][lane]) + s_part[1][1][lane];
+            fz = (fz + s_part[0][2][lane]) + s_part[1][2][lane];
+            S.frc[nxt][s] = make_float4(fx, fy, fz, 0.0f);
+        }
+        __syncthreads();
     }
 }
 
@@ -442,74 +403,81 @@ __device__ __forceinline__ float min_image(float d, float len)
     return d;
 }
 
-__global__ __launch_bounds__(kBeadThreads) void k_bonds(DeviceState S, int nxt, int par, uint32_t n_polys,
+// One thread per bonded bead: its Hookean and angle terms, each recomputed from
+// the positions of the term's beads exactly as the per-polymer loop computes
+// it, and added in that loop's order (bonds, then bond pairs; within a bond pair
+// head, tail, middle), so the sum equals the per-polymer sequence bit for bit.
+__global__ __launch_bounds__(kBeadThreads) void k_bonds(DeviceState S, int nxt, int par, uint32_t n_bonded,
                                                        uint32_t n_worlds)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const WorldCtr *__restrict__ ctr_in = S.ctr + par * n_worlds;
-    if (p < n_polys) {
-        const PolyItem it = S.polys[p];
-        if (ctr_in[it.world].remaining > 0) {
-            const WorldDev *__restrict__ W = &S.worlds[it.world];
+    if (i < n_bonded) {
+        const BondedBead bb = S.bonded[i];
+        if (ctr_in[bb.world].remaining > 0) {
+            const WorldDev *__restrict__ W = &S.worlds[bb.world];
             const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
-            float4 *__restrict__ frc = S.frc[nxt];
             const float4 *__restrict__ pos = S.pos[nxt];
-            for (uint32_t bi = it.bond_begin; bi < it.bond_end; bi++) {
-                const BondRec b = S.bonds[bi];
-                const uint32_t sh = S.slot_of_bead[b.head], st = S.slot_of_bead[b.tail];
-                const float4 xh = pos[sh], xt = pos[st];
-                const float dx = min_image(xt.x - xh.x, Lx);
-                const float dy = min_image(xt.y - xh.y, Ly);
-                const float dz = min_image(xt.z - xh.z, Lz);
-                const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-                const float hf = b.kappa * (b.r0 - r);
-                const float s = hf / r;
-                float4 fh = frc[sh];
-                fh.x -= dx * s; fh.y -= dy * s; fh.z -= dz * s;
-                frc[sh] = fh;
-                float4 ft = frc[st];
-                ft.x += dx * s; ft.y += dy * s; ft.z += dz * s;
-                frc[st] = ft;
-            }
-            for (uint32_t ai = it.angle_begin; ai < it.angle_end; ai++) {
-                const AngleRec a = S.angles[ai];
-                const uint32_t sh = S.slot_of_bead[a.head], sm = S.slot_of_bead[a.mid], st = S.slot_of_bead[a.tail];
-                const float4 xh = pos[sh], xm = pos[sm], xt = pos[st];
-                const float d0x = min_image(xm.x - xh.x, Lx), d0y = min_image(xm.y - xh.y, Ly),
-                            d0z = min_image(xm.z - xh.z, Lz);
-                const float d1x = min_image(xt.x - xm.x, Lx), d1y = min_image(xt.y - xm.y, Ly),
-                            d1z = min_image(xt.z - xm.z, Lz);
-                const float r01 = sqrtf(fmaf(d0z, d0z, fmaf(d0y, d0y, d0x * d0x)));
-                const float r12 = sqrtf(fmaf(d1z, d1z, fmaf(d1y, d1y, d1x * d1x)));
-                const float mag = r01 * r12;
-                float hx = 0.f, hy = 0.f, hz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
-                if (mag > 0.0001f) {
-                    const float b1 = r01 * r01, b2 = r12 * r12;
-                    const float dot = fmaf(d0z, d1z, fmaf(d0y, d1y, d0x * d1x));
-                    const float o1 = dot / b1, o2 = dot / b2;
-                    const float cos2 = o1 * o2;
-                    float fm = a.kappa / mag;
-                    if (fabsf(dot) > 0.000001f && a.sin0 > 0.0f) {
-                        const float ip = 1.0f / sqrtf(1.0f / cos2 - 1.0f);
-                        fm = fm * (a.cos0 - a.sin0 * ip);
+            const uint32_t me = S.slot_of_bead[bb.bead];
+            float4 f = S.frc[nxt][me];
+            for (uint32_t t = bb.t0; t < bb.t1; t++) {
+                const uint32_t code = S.bead_terms[t];
+                const uint32_t role = code >> kRoleShift;
+                const uint32_t idx = code & ((1u << kRoleShift) - 1u);
+                if (role <= kRoleBondTail) {
+                    const BondRec b = S.bonds[idx];
+                    const float4 xh = pos[S.slot_of_bead[b.head]], xt = pos[S.slot_of_bead[b.tail]];
+                    const float dx = min_image(xt.x - xh.x, Lx);
+                    const float dy = min_image(xt.y - xh.y, Ly);
+                    const float dz = min_image(xt.z - xh.z, Lz);
+                    const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+                    const float hf = b.kappa * (b.r0 - r);
+                    const float sc = hf / r;
+                    if (role == kRoleBondHead) { f.x -= dx * sc; f.y -= dy * sc; f.z -= dz * sc; }
+                    else { f.x += dx * sc; f.y += dy * sc; f.z += dz * sc; }
+                } else {
+                    const AngleRec a = S.angles[idx];
+                    const float4 xh = pos[S.slot_of_bead[a.head]], xm = pos[S.slot_of_bead[a.mid]],
+                                 xt = pos[S.slot_of_bead[a.tail]];
+                    const float d0x = min_image(xm.x - xh.x, Lx), d0y = min_image(xm.y - xh.y, Ly),
+                                d0z = min_image(xm.z - xh.z, Lz);
+                    const float d1x = min_image(xt.x - xm.x, Lx), d1y = min_image(xt.y - xm.y, Ly),
+                                d1z = min_image(xt.z - xm.z, Lz);
+                    const float r01 = sqrtf(fmaf(d0z, d0z, fmaf(d0y, d0y, d0x * d0x)));
+                    const float r12 = sqrtf(fmaf(d1z, d1z, fmaf(d1y, d1y, d1x * d1x)));
+                    const float mag = r01 * r12;
+                    float hx = 0.f, hy = 0.f, hz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
+                    if (mag > 0.0001f) {
+                        const float b1 = r01 * r01, b2 = r12 * r12;
+                        const float dot = fmaf(d0z, d1z, fmaf(d0y, d1y, d0x * d1x));
+                        const float o1 = dot / b1, o2 = dot / b2;
+                        const float cos2 = o1 * o2;
+                        float fm = a.kappa / mag;
+                        if (fabsf(dot) > 0.000001f && a.sin0 > 0.0f) {
+                            // |cot phi| with the naive engine's floor on the prefactor
+                            // (naive_dpd_engine.hpp:681-682); the argument is clamped at 0
+                            // because FP32 rounding can push cos^2 phi just above 1
+                            const float pref = fmaxf(sqrtf(fmaxf(1.0f / cos2 - 1.0f, 0.0f)), 0.000001f);
+                            const float ip = 1.0f / pref;
+                            fm = fm * (a.cos0 - a.sin0 * ip);
+                        }
+                        hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
+                        tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
                     }
-                    hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
-                    tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
-                    mx = -(hx + tx); my = -(hy + ty); mz = -(hz + tz);
+                    if (role == kRoleAngleHead) { f.x += hx; f.y += hy; f.z += hz; }
+                    else if (role == kRoleAngleTail) { f.x += tx; f.y += ty; f.z += tz; }
+                    else { f.x += -(hx + tx); f.y += -(hy + ty); f.z += -(hz + tz); }
                 }
-                float4 f;
-                f = frc[sh]; f.x += hx; f.y += hy; f.z += hz; frc[sh] = f;
-                f = frc[st]; f.x += tx; f.y += ty; f.z += tz; frc[st] = f;
-                f = frc[sm]; f.x += mx; f.y += my; f.z += mz; frc[sm] = f;
             }
+            S.frc[nxt][me] = f;
         }
     }
     // step counters: read ctr[par], write ctr[par^1] (nothing reads that copy this step)
-    if (p < n_worlds) {
-        const WorldCtr c = S.ctr[par * n_worlds + p];
+    if (i < n_worlds) {
+        const WorldCtr c = S.ctr[par * n_worlds + i];
         WorldCtr o = c;
         if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
-        S.ctr[(par ^ 1) * n_worlds + p] = o;
+        S.ctr[(par ^ 1) * n_worlds + i] = o;
     }
 }
 
@@ -535,6 +503,8 @@ __global__ void k_rebase_index(float4 *__restrict__ vel, uint32_t n, uint32_t de
 
 static inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+uint32_t scan_tiles(uint32_t n_cells) { return std::max<uint32_t>(1, ceil_div(n_cells, kScanBlock)); }
+
 hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
 {
     const DeviceState &S = L.S;
@@ -543,25 +513,24 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     DeviceState Sp = S;
     Sp.ctr = S.ctr + par * L.n_worlds;       // counters read this step
     auto rec = [&](int k) { if (ev) (void)hipEventRecord(ev->ev[k], st); };
+    const uint32_t nt = scan_tiles(L.n_cells);
 
     rec(0);
-    k_integrate<<<L.n_bead_blocks, kBeadThreads, 0, st>>>(Sp, cur);
+    k_integrate<<<std::max<uint32_t>(1, L.n_bead_blocks), kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks);
     rec(1);
-    const uint32_t nb = ceil_div(L.n_cells, kScanBlock);
-    k_scan_reduce<<<nb, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.block_sums);
-    k_scan_top<<<1, kScanThreads, 0, st>>>(S.block_sums, nb, S.cell_start, L.n_cells);
-    k_scan_down<<<nb, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.block_sums, S.cell_start);
+    k_scan<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, S.scan_tiles, S.scan_ticket, nt,
+                                         S.flags);
     rec(2);
     const uint32_t gb = ceil_div(L.n_beads, kBeadThreads);
-    k_scatter<<<gb, kBeadThreads, 0, st>>>(S, L.n_beads);
+    k_scatter<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
     rec(3);
     k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
     rec(4);
-    const uint32_t fg = L.force_blocks ? std::min<uint32_t>(L.force_blocks, L.n_bricks) : L.n_bricks;
-    if (L.n_bricks) k_force<<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_bricks);
+    const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
+    if (fg) k_force<<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks);
     rec(5);
-    const uint32_t pw = std::max<uint32_t>(L.n_polys, L.n_worlds);
-    k_bonds<<<ceil_div(pw, kBeadThreads), kBeadThreads, 0, st>>>(S, cur ^ 1, par, L.n_polys, L.n_worlds);
+    const uint32_t pw = std::max<uint32_t>(L.n_bonded, L.n_worlds);
+    k_bonds<<<ceil_div(pw, kBeadThreads), kBeadThreads, 0, st>>>(S, cur ^ 1, par, L.n_bonded, L.n_worlds);
     rec(6);
     return hipGetLastError();
 }

@@ -43,21 +43,20 @@ class WorldDesc(C.Structure):
 
 class Limits(C.Structure):
     _fields_ = [("max_bead_types", C.c_uint32), ("min_box", C.c_uint32), ("max_box", C.c_uint32),
-                ("max_polymer_length", C.c_uint32), ("tile_capacity", C.c_uint32),
+                ("max_polymer_length", C.c_uint32), ("max_beads", C.c_uint32),
                 ("max_bonds_per_bead", C.c_uint32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("upload_s", C.c_double), ("run_s", C.c_double), ("steps", C.c_uint64),
                 ("bead_steps", C.c_uint64), ("flags", C.c_uint32), ("n_worlds", C.c_uint32),
-                ("n_beads", C.c_uint64), ("n_bricks", C.c_uint32), ("brick", C.c_uint32 * 3),
+                ("n_beads", C.c_uint64), ("n_force_blocks", C.c_uint32),
                 ("k_integrate_ms", C.c_double), ("k_scan_ms", C.c_double), ("k_scatter_ms", C.c_double),
                 ("k_reorder_ms", C.c_double), ("k_force_ms", C.c_double), ("k_bonds_ms", C.c_double),
                 ("profiled_steps", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
-        d["brick"] = tuple(self.brick)
         return d
 
 
