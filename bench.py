@@ -123,6 +123,19 @@ def run_batched(args, ctx_cls, W):
             "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "tuning": tune}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counters give the scale from counter KiB to bytes moved."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,12 +149,20 @@ def main():
     ap.add_argument("--batch-solo-sample", type=int, default=32)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--side", type=int, default=32)
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of this many MiB, 5 times (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(independent worlds shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from pdpd_amd import capi
     from pdpd_amd import world as W
@@ -152,6 +173,8 @@ def main():
     log(f"bench: polymer world {args.side}^3, {n} beads; warmup {args.warmup}, timed {args.steps}")
     ctx = capi.Context(0)
     slot = ctx.add(world)
+    if args.graph_steps is not None:
+        ctx.set_config("graph_steps", str(args.graph_steps))
     if args.autotune:
         log("bench: autotune " + ctx.autotune(20))
     ctx.run(args.warmup)

@@ -190,6 +190,7 @@ class HipDPDEngine : public DPDEngine {
 protected:
     bool CanSupportHookeanBonds() const override { return true; }
     bool CanSupportAngleBonds() const override { return true; }
+    bool CanSupportStationaryBeadTypes() const override { return true; }
 
 public:
     explicit HipDPDEngine(int device = 0, unsigned graph_steps = 20) : m_ctx(device), m_graph_steps(graph_steps) {}
@@ -397,7 +398,7 @@ public:
         throw std::runtime_error("HipBatchDPDEngine: unknown slot");
     }
 
-    // Let the autotuner pick the brick and launch shape for the attached worlds.
+    // Let the autotuner pick the launch shape (force grid, steps per graph) for the attached worlds.
     std::string Autotune(unsigned steps_per_trial)
     {
         char buf[256] = {0};

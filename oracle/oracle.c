@@ -338,11 +338,21 @@ int orc_naive_run(orc_world *w, int nsteps)
  *   x' = x + dt*v'             (drift, raw:54-58, written as (f*dt/2 + v)*dt + x)
  *   wrap: x += (x<0?L:0); x -= (x>=L?L:0)   (raw:61-68)
  * w is replaced by v'. */
+static int f32_stationary(const orc_f32_world *w, int i)
+{
+    return w->stationary && w->stationary[w->hash[i] >> 28];
+}
+
 void orc_f32_integrate(orc_f32_world *w)
 {
     const float dt = w->dt;
     const float hdt = 0.5f * dt;
     for (int i = 0; i < w->n_beads; i++) {
+        if (f32_stationary(w, i)) {
+            /* x stays; the pair force sees v(t) = w + f dt/2 (naive_dpd_engine_half_merge.hpp:327-331) */
+            for (int d = 0; d < 3; d++) w->w[3 * i + d] = fmaf(w->f[3 * i + d], hdt, w->w[3 * i + d]);
+            continue;
+        }
         for (int d = 0; d < 3; d++) {
             float Lf = (float)w->L[d];
             float fd = w->f[3 * i + d];
@@ -425,6 +435,12 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
     const float scale = (float)sqrt(24.0 / (double)w->dt);
 
     for (int i = 0; i < n; i++) {
+        if (f32_stationary(w, i)) {
+            /* a stationary bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390) */
+            f_out[3 * i] = f_out[3 * i + 1] = f_out[3 * i + 2] = 0.0f;
+            if (fabs_sum) fabs_sum[i] = 0.0f;
+            continue;
+        }
         int hc[3];
         cell_key_f32(w, i, hc);
         const float hx = w->x[3 * i], hy = w->x[3 * i + 1], hz = w->x[3 * i + 2];
@@ -488,75 +504,86 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
     free(cs);
 }
 
-static void min_image_f32(const int *L, const float *base, const float *other, float *res)
+static double min_image_f64(double d, double len)
 {
-    for (int d = 0; d < 3; d++) {
-        float len = (float)L[d];
-        float dx = other[d] - base[d];
-        if (dx > 0.5f * len) dx -= len;
-        else if (dx < -0.5f * len) dx += len;
-        res[d] = dx;
-    }
+    if (d > 0.5 * len) d -= len;
+    else if (d < -0.5 * len) d += len;
+    return d;
 }
 
-static float norm3_f32(const float *v)
+/* Difference b - a of FP32 coordinates, exact in double, wrapped to the minimum image. */
+static void diff_f64(const int *L, const float *a, const float *b, double *res)
 {
-    return sqrtf(fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0])));
+    for (int d = 0; d < 3; d++) res[d] = min_image_f64((double)b[d] - (double)a[d], (double)L[d]);
 }
 
+static double norm3_f64(const double *v)
+{
+    return sqrt(fma(v[2], v[2], fma(v[1], v[1], v[0] * v[0])));
+}
+
+/* Bonded terms in FP64 from the FP32 positions. Each bead's terms are added to
+ * its FP32 pair force in a double accumulator, in the order the per-polymer loop
+ * applies them (the polymer's bonds, then its bond pairs), and rounded to float
+ * once. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
+ * dpd_maths_core_half_step.hpp:178-231 via naive_dpd_engine_half_merge.hpp:656-690,
+ * with the prefactor floor of naive_dpd_engine.hpp:681-682 (argument clamped at 0). */
 void orc_f32_bonds(const orc_f32_world *w, float *F)
 {
+    int n = w->n_beads;
+    double *acc = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    unsigned char *touched = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    for (int i = 0; i < 3 * n; i++) acc[i] = (double)F[i];
     for (int p = 0; p < w->n_polymers; p++) {
-        /* Hookean: naive_dpd_engine_half_merge.hpp:638-654 */
         for (int bi = w->poly_bond_start[p]; bi < w->poly_bond_start[p + 1]; bi++) {
             int h = w->bond_head[bi], t = w->bond_tail[bi];
-            float dx[3];
-            min_image_f32(w->L, &w->x[3 * h], &w->x[3 * t], dx);
-            float r = norm3_f32(dx);
-            float hf = w->bond_kappa[bi] * (w->bond_r0[bi] - r);
-            float s = hf / r;
+            double dx[3];
+            diff_f64(w->L, &w->x[3 * h], &w->x[3 * t], dx);
+            double r = norm3_f64(dx);
+            double hf = w->bond_kappa[bi] * (w->bond_r0[bi] - r);
+            double sc = hf / r;
             for (int d = 0; d < 3; d++) {
-                float fd = dx[d] * s;
-                F[3 * h + d] -= fd;
-                F[3 * t + d] += fd;
+                acc[3 * h + d] += -(dx[d] * sc);
+                acc[3 * t + d] += dx[d] * sc;
             }
+            touched[h] = touched[t] = 1;
         }
-        /* Angle: dpd_maths_core_half_step.hpp:178-231 via
-         * naive_dpd_engine_half_merge.hpp:656-690 */
         for (int ai = w->poly_angle_start[p]; ai < w->poly_angle_start[p + 1]; ai++) {
             int h = w->angle_head[ai], m = w->angle_mid[ai], t = w->angle_tail[ai];
-            float d01[3], d12[3];
-            min_image_f32(w->L, &w->x[3 * h], &w->x[3 * m], d01);
-            min_image_f32(w->L, &w->x[3 * m], &w->x[3 * t], d12);
-            float r01 = norm3_f32(d01), r12 = norm3_f32(d12);
-            float mag = r01 * r12;
-            float hf[3] = {0, 0, 0}, tf[3] = {0, 0, 0}, mf[3] = {0, 0, 0};
-            if (mag > 0.0001f) {
-                float b1 = r01 * r01, b2 = r12 * r12;
-                float dot = fmaf(d01[2], d12[2], fmaf(d01[1], d12[1], d01[0] * d12[0]));
-                float o1 = dot / b1, o2 = dot / b2;
-                float cos2 = o1 * o2;
-                float fm = w->angle_kappa[ai] / mag;
-                float s0 = w->angle_sin0[ai];
-                if (fabsf(dot) > 0.000001f && s0 > 0.0f) {
-                    /* prefactor floor of naive_dpd_engine.hpp:681-682; argument clamped at 0 */
-                    float pref = fmaxf(sqrtf(fmaxf(1.0f / cos2 - 1.0f, 0.0f)), 0.000001f);
-                    float ip = 1.0f / pref;
-                    fm = fm * (w->angle_cos0[ai] - s0 * ip);
+            double d01[3], d12[3];
+            diff_f64(w->L, &w->x[3 * h], &w->x[3 * m], d01);
+            diff_f64(w->L, &w->x[3 * m], &w->x[3 * t], d12);
+            double r01 = norm3_f64(d01), r12 = norm3_f64(d12);
+            double mag = r01 * r12;
+            double hf[3] = {0, 0, 0}, tf[3] = {0, 0, 0};
+            if (mag > 0.0001) {
+                double b1 = r01 * r01, b2 = r12 * r12;
+                double dot = fma(d01[2], d12[2], fma(d01[1], d12[1], d01[0] * d12[0]));
+                double o1 = dot / b1, o2 = dot / b2;
+                double cos2 = o1 * o2;
+                double fm = w->angle_kappa[ai] / mag;
+                if (fabs(dot) > 0.000001 && w->angle_sin0[ai] > 0.0) {
+                    double pref = fmax(sqrt(fmax(1.0 / cos2 - 1.0, 0.0)), 0.000001);
+                    fm = fm * (w->angle_cos0[ai] - w->angle_sin0[ai] * (1.0 / pref));
                 }
                 for (int d = 0; d < 3; d++) {
                     hf[d] = (d01[d] * o1 - d12[d]) * fm;
                     tf[d] = (d01[d] - d12[d] * o2) * fm;
-                    mf[d] = -(hf[d] + tf[d]);
                 }
             }
             for (int d = 0; d < 3; d++) {
-                F[3 * h + d] += hf[d];
-                F[3 * t + d] += tf[d];
-                F[3 * m + d] += mf[d];
+                acc[3 * h + d] += hf[d];
+                acc[3 * t + d] += tf[d];
+                acc[3 * m + d] += -(hf[d] + tf[d]);
             }
+            touched[h] = touched[m] = touched[t] = 1;
         }
     }
+    for (int i = 0; i < n; i++)
+        if (touched[i] && !f32_stationary(w, i))
+            for (int d = 0; d < 3; d++) F[3 * i + d] = (float)acc[3 * i + d];
+    free(acc);
+    free(touched);
 }
 
 int orc_f32_step(orc_f32_world *w, int32_t *order, int32_t *cell_start, int32_t *key, float *fabs_sum)
@@ -567,6 +594,9 @@ int orc_f32_step(orc_f32_world *w, int32_t *order, int32_t *cell_start, int32_t 
     uint64_t t_hash = orc_t_hash(w->t, w->seed);
     orc_f32_pair_forces(w, t_hash, w->f, fabs_sum);
     orc_f32_bonds(w, w->f);
+    /* stationary beads end the step with v = 0 (naive_dpd_engine_half_merge.hpp:387-390) */
+    for (int i = 0; i < w->n_beads; i++)
+        if (f32_stationary(w, i)) w->w[3 * i] = w->w[3 * i + 1] = w->w[3 * i + 2] = 0.0f;
     w->t += 1;
     return 0;
 }

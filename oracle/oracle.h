@@ -114,9 +114,10 @@ typedef struct orc_f32_world {
     int32_t n_polymers;
     const int32_t *poly_bond_start, *poly_angle_start;
     const int32_t *bond_head, *bond_tail;
-    const float *bond_kappa, *bond_r0;
+    const double *bond_kappa, *bond_r0;    /* bonded terms are evaluated in FP64 */
     const int32_t *angle_head, *angle_mid, *angle_tail;
-    const float *angle_kappa, *angle_cos0, *angle_sin0;
+    const double *angle_kappa, *angle_cos0, *angle_sin0;
+    const uint8_t *stationary;             /* n_types, may be NULL */
 } orc_f32_world;
 
 int orc_f32_step(orc_f32_world *w, int32_t *order, int32_t *cell_start, int32_t *key, float *fabs_sum);

@@ -5,6 +5,7 @@ __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -38,6 +39,7 @@ class OrcF32World(C.Structure):
         ("bond_head", C.c_void_p), ("bond_tail", C.c_void_p), ("bond_kappa", C.c_void_p), ("bond_r0", C.c_void_p),
         ("angle_head", C.c_void_p), ("angle_mid", C.c_void_p), ("angle_tail", C.c_void_p),
         ("angle_kappa", C.c_void_p), ("angle_cos0", C.c_void_p), ("angle_sin0", C.c_void_p),
+        ("stationary", C.c_void_p),
     ]
 
 
@@ -182,10 +184,14 @@ class F32State:
         topo = world.flat_topology()
         self.keep = {k: v for k, v in topo.items()}
         for k in ("bond_kappa", "bond_r0", "angle_kappa"):
-            self.keep[k] = np.ascontiguousarray(topo[k], np.float32)
-        self.keep["angle_cos0"] = np.ascontiguousarray(np.cos(topo["angle_theta0"]), np.float32)
-        self.keep["angle_sin0"] = np.ascontiguousarray(np.sin(topo["angle_theta0"]), np.float32)
+            self.keep[k] = np.ascontiguousarray(topo[k], np.float64)
+        # the C library's cos/sin, as the engine's host code uses
+        th = np.asarray(topo["angle_theta0"], np.float64)
+        self.keep["angle_cos0"] = np.array([math.cos(a) for a in th], np.float64)
+        self.keep["angle_sin0"] = np.array([math.sin(a) for a in th], np.float64)
         self.keep["hash"] = np.ascontiguousarray(world.hash, np.uint32)
+        self.keep["stat"] = (np.ascontiguousarray(world.stationary, np.uint8)
+                             if world.stationary is not None else None)
         self.order = None
         self.cell_start = None
         self.key = None
@@ -209,6 +215,7 @@ class F32State:
         for name in ("poly_bond_start", "poly_angle_start", "bond_head", "bond_tail", "bond_kappa", "bond_r0",
                      "angle_head", "angle_mid", "angle_tail", "angle_kappa", "angle_cos0", "angle_sin0"):
             setattr(s, name, _p(k[name]))
+        s.stationary = _p(k["stat"])
         return s
 
     def step(self, nsteps=1):

@@ -78,11 +78,13 @@ struct WorldHost {
     uint32_t budget = UINT32_MAX; // steps allowed in the next run
     uint32_t n = 0;
     uint32_t n_types = 0;
+    uint32_t stationary = 0;      // bit per stationary bead type
     float a[kMaxTypes * kMaxTypes] = {};
     float sg[kMaxTypes * kMaxTypes] = {};
     std::vector<float4> up_pos, up_vel, up_frc;   // pending upload (bead order)
     std::vector<BondRec> bonds;                    // local bead indices, reference order
     std::vector<AngleRec> angles;
+    std::vector<uint32_t> stationary_beads;        // local indices of beads of stationary types
     uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
 };
@@ -137,11 +139,13 @@ struct pdpd_ctx {
     DevBuf<WorldCtr> ctr;
     DevBuf<BeadBlock> bblocks, fblocks;
     DevBuf<BondedBead> bonded;
+    DevBuf<BondBlock> bond_blocks;
+    DevBuf<StationaryBead> stationary;
     DevBuf<uint32_t> bead_terms;
     DevBuf<BondRec> bonds;
     DevBuf<AngleRec> angles;
     int cur = 0;
-    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_bonded = 0;
+    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_bond_blocks = 0, n_stationary = 0;
     std::vector<int> order;               // slots in layout order
     std::vector<WorldCtr> ctr_host;
 
@@ -163,8 +167,8 @@ struct pdpd_ctx {
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
         key.release(); rank.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
-        wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); bonded.release(); bead_terms.release();
-        bonds.release(); angles.release();
+        wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); bonded.release(); bond_blocks.release();
+        stationary.release(); bead_terms.release(); bonds.release(); angles.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -177,7 +181,8 @@ struct pdpd_ctx {
         S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
         S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
         S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
-        S.bonded = bonded.p; S.bead_terms = bead_terms.p; S.bonds = bonds.p; S.angles = angles.p;
+        S.bonded = bonded.p; S.bond_blocks = bond_blocks.p; S.bead_terms = bead_terms.p;
+        S.bonds = bonds.p; S.angles = angles.p; S.stationary_beads = stationary.p;
         return S;
     }
 
@@ -187,7 +192,8 @@ struct pdpd_ctx {
         L.S = state();
         L.cur = c;
         L.n_beads = n_beads; L.n_cells = n_cells; L.n_worlds = n_worlds;
-        L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks; L.n_bonded = n_bonded;
+        L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks; L.n_bond_blocks = n_bond_blocks;
+        L.n_stationary = n_stationary;
         L.force_grid = force_grid;
         return L;
     }
@@ -330,6 +336,7 @@ int relayout(pdpd_ctx *ctx)
         d.cell_off = w.cell_off; d.n_cells = w.n_cells();
         d.seed_lo = (uint32_t)w.seed; d.seed_hi = (uint32_t)(w.seed >> 32);
         d.n_types = w.n_types;
+        d.stationary = w.stationary;
         memcpy(d.a, w.a, sizeof(d.a));
         memcpy(d.sg, w.sg, sizeof(d.sg));
     }
@@ -349,6 +356,7 @@ int relayout(pdpd_ctx *ctx)
     std::vector<BondRec> brec;
     std::vector<AngleRec> arec;
     std::vector<BondedBead> bbead;
+    std::vector<BondBlock> bblk;
     std::vector<uint32_t> terms;
     for (size_t i = 0; i < ws.size(); i++) {
         const WorldHost &w = *ws[i];
@@ -370,19 +378,41 @@ int relayout(pdpd_ctx *ctx)
             r.head += w.bead_off; r.mid += w.bead_off; r.tail += w.bead_off;
             arec.push_back(r);
         }
+        // groups of whole bonded beads of this world, at most kBondThreads terms each
+        bool open = false;
         for (uint32_t b = 0; b < w.n; b++) {
             if (per[b].empty()) continue;
+            if (per[b].size() > (size_t)kBondThreads)
+                return fail(PDPD_ERR_INVALID, "a bead takes part in more than " + std::to_string(kBondThreads) +
+                                                  " bonds and angles");
+            if (!open || terms.size() + per[b].size() - bblk.back().term_begin > (size_t)kBondThreads) {
+                BondBlock k{};
+                k.bead_begin = k.bead_end = (uint32_t)bbead.size();
+                k.term_begin = k.term_end = (uint32_t)terms.size();
+                k.world = (int32_t)i;
+                bblk.push_back(k);
+                open = true;
+            }
             BondedBead e{w.bead_off + b, (uint32_t)terms.size(), 0, (int32_t)i};
             terms.insert(terms.end(), per[b].begin(), per[b].end());
             e.t1 = (uint32_t)terms.size();
             bbead.push_back(e);
+            bblk.back().bead_end = (uint32_t)bbead.size();
+            bblk.back().term_end = (uint32_t)terms.size();
         }
     }
     if (brec.size() + arec.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
-    ctx->n_bonded = (uint32_t)bbead.size();
+    ctx->n_bond_blocks = (uint32_t)bblk.size();
     HIP_TRY(dev_upload(ctx->bonds, brec));
     HIP_TRY(dev_upload(ctx->angles, arec));
     HIP_TRY(dev_upload(ctx->bonded, bbead));
+    HIP_TRY(dev_upload(ctx->bond_blocks, bblk));
+
+    std::vector<StationaryBead> sb;
+    for (size_t i = 0; i < ws.size(); i++)
+        for (uint32_t b : ws[i]->stationary_beads) sb.push_back({ws[i]->bead_off + b, (int32_t)i});
+    ctx->n_stationary = (uint32_t)sb.size();
+    HIP_TRY(dev_upload(ctx->stationary, sb));
     HIP_TRY(dev_upload(ctx->bead_terms, terms));
 
     ctx->stats.n_worlds = ctx->n_worlds;
@@ -515,9 +545,21 @@ std::string support_reason(const pdpd_world_desc *w)
     if (!(w->dt > 0)) return "dt must be positive.";
     if (w->n_bead_types == 0 || w->n_bead_types > (uint32_t)kMaxTypes)
         return "Engine supports 1 to " + std::to_string(kMaxTypes) + " bead types.";
-    if (w->stationary) {
-        for (uint32_t i = 0; i < w->n_bead_types; i++)
-            if (w->stationary[i]) return "Engine does not support stationary bead types.";
+    // bonded terms per bead (per polymer-type offset): the bond kernel gives each
+    // term of a bead a lane of one workgroup
+    for (uint32_t pt = 0; pt < w->n_polymer_types; pt++) {
+        uint32_t cnt[kMaxPolymerLength + 1] = {};
+        const uint32_t b0 = w->pt_bond_start[pt], b1 = w->pt_bond_start[pt + 1];
+        auto add = [&](uint32_t off) { if (off <= kMaxPolymerLength) cnt[off]++; };
+        for (uint32_t b = b0; b < b1; b++) { add(w->bond_head[b]); add(w->bond_tail[b]); }
+        for (uint32_t a = w->pt_angle_start[pt]; a < w->pt_angle_start[pt + 1]; a++) {
+            const uint32_t hb = b0 + w->angle_head_bond[a], tb = b0 + w->angle_tail_bond[a];
+            if (hb >= b1 || tb >= b1) continue;      // rejected with a message at import
+            add(w->bond_head[hb]); add(w->bond_tail[hb]); add(w->bond_tail[tb]);
+        }
+        for (uint32_t k = 0; k <= kMaxPolymerLength; k++)
+            if (cnt[k] > (uint32_t)kBondThreads)
+                return "A bead takes part in more than " + std::to_string(kBondThreads) + " bonds and angles.";
     }
     for (uint32_t p = 0; p < w->n_polymers; p++) {
         uint32_t len = w->polymer_bead_start[p + 1] - w->polymer_bead_start[p];
@@ -558,7 +600,7 @@ int pdpd_get_limits(pdpd_limits *out)
     out->max_box = kMaxBox;
     out->max_polymer_length = kMaxPolymerLength;
     out->max_beads = kMaxLayoutBeads - 1;
-    out->max_bonds_per_bead = 0xffffffffu;
+    out->max_bonds_per_bead = kBondThreads;
     return PDPD_OK;
 }
 
@@ -606,6 +648,9 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
     w.n = d->n_beads;
     w.n_types = d->n_bead_types;
     const uint32_t nt = d->n_bead_types;
+    if (d->stationary)
+        for (uint32_t i = 0; i < nt; i++)
+            if (d->stationary[i]) w.stationary |= 1u << i;
     for (uint32_t i = 0; i < nt; i++)
         for (uint32_t j = 0; j < nt; j++) {
             w.a[i * kMaxTypes + j] = (float)d->conservative[i * nt + j];
@@ -628,8 +673,14 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
             f[k] = (float)d->f[3 * i + k];
             v[k] = std::fmaf(f[k], -hdt, v[k]);
         }
+        if (d->bead_type[i] >= nt) return fail(PDPD_ERR_INVALID, "bead " + std::to_string(i) + ": bead type out of range");
+        // BeadHash fields (dpd_state.hpp:18-35): 27-bit monomer id, or 20-bit polymer id + 7-bit offset
+        if (d->is_monomer[i] ? d->polymer_id[i] >= (1u << 27)
+                             : (d->polymer_id[i] >= (1u << 20) || d->polymer_offset[i] > kMaxPolymerLength))
+            return fail(PDPD_ERR_INVALID, "bead " + std::to_string(i) + ": polymer id or offset does not fit the bead hash");
         const uint32_t h = bead_hash(d->bead_type[i], d->is_monomer[i] != 0, d->polymer_id[i], d->polymer_offset[i]);
         w.up_pos[i] = make_float4(x[0], x[1], x[2], u2f(h));
+        if ((w.stationary >> d->bead_type[i]) & 1u) w.stationary_beads.push_back(i);
         w.up_vel[i] = make_float4(v[0], v[1], v[2], 0.0f);
         w.up_frc[i] = make_float4(f[0], f[1], f[2], 0.0f);
     }
@@ -641,7 +692,7 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
         const uint32_t len = d->polymer_bead_start[p + 1] - d->polymer_bead_start[p];
         for (uint32_t b = d->pt_bond_start[pt]; b < d->pt_bond_start[pt + 1]; b++) {
             if (d->bond_head[b] >= len || d->bond_tail[b] >= len) return fail(PDPD_ERR_INVALID, "bond offset out of range");
-            w.bonds.push_back({ids[d->bond_head[b]], ids[d->bond_tail[b]], (float)d->bond_kappa[b], (float)d->bond_r0[b]});
+            w.bonds.push_back({ids[d->bond_head[b]], ids[d->bond_tail[b]], d->bond_kappa[b], d->bond_r0[b]});
         }
         for (uint32_t a = d->pt_angle_start[pt]; a < d->pt_angle_start[pt + 1]; a++) {
             const uint32_t hb = d->pt_bond_start[pt] + d->angle_head_bond[a];
@@ -652,9 +703,9 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
             r.head = ids[d->bond_head[hb]];
             r.mid = ids[d->bond_tail[hb]];
             r.tail = ids[d->bond_tail[tb]];
-            r.kappa = (float)d->angle_kappa[a];
-            r.cos0 = (float)std::cos(d->angle_theta0[a]);
-            r.sin0 = (float)std::sin(d->angle_theta0[a]);
+            r.kappa = d->angle_kappa[a];
+            r.cos0 = std::cos(d->angle_theta0[a]);
+            r.sin0 = std::sin(d->angle_theta0[a]);
             w.angles.push_back(r);
         }
     }

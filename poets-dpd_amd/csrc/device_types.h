@@ -13,7 +13,7 @@ constexpr int kForceBeads = 64;        // beads per pair-force workgroup (one pe
 constexpr int kForcePlanes = 3;        // waves per pair-force workgroup: one per x-plane of the stencil
 constexpr int kForceThreads = kForceBeads * kForcePlanes;
 constexpr int kPairList = 16;          // accepted candidates buffered per lane before evaluation
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 1024;
 constexpr int kScanPerThread = 8;
 constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan tile
 constexpr uint32_t kMaxLayoutBeads = 1u << 26;              // pair-list entries carry 26-bit slots
@@ -30,7 +30,8 @@ struct WorldDev {
     uint32_t n_cells;
     uint32_t seed_lo, seed_hi;
     uint32_t n_types;
-    uint32_t pad0[3];
+    uint32_t stationary;   // bit k set: bead type k is stationary (never moves, v = f = 0 after a step)
+    uint32_t pad0[2];
     float a[kMaxTypes * kMaxTypes];    // conservative a_ij
     float sg[kMaxTypes * kMaxTypes];   // sqrt(gamma_ij)
 };
@@ -50,15 +51,24 @@ struct BeadBlock {
 
 // Bonded topology. Bond and angle records hold global bead indices, in the
 // reference's order (per polymer: the type's bonds, then its bond pairs).
-struct BondRec { uint32_t head, tail; float kappa, r0; };
-struct AngleRec { uint32_t head, mid, tail, pad; float kappa, cos0, sin0, pad1; };
+// Bonded terms are evaluated in FP64 (see k_bonds).
+struct BondRec { uint32_t head, tail; double kappa, r0; };
+struct AngleRec { uint32_t head, mid, tail, pad; double kappa, cos0, sin0; };
 
 // A bead that takes part in bonds: its terms are bead_terms[t0, t1), each
 // (role << 29) | record index, in the order the per-polymer loop applies them.
 struct BondedBead { uint32_t bead; uint32_t t0, t1; int32_t world; };
+
+// A group of whole bonded beads with at most kBondThreads terms: one lane per
+// term, then one lane per bead sums its terms in order.
+constexpr int kBondThreads = 256;
+struct BondBlock { uint32_t bead_begin, bead_end, term_begin, term_end; int32_t world; uint32_t pad[3]; };
 constexpr uint32_t kRoleShift = 29;
 constexpr uint32_t kRoleBondHead = 0, kRoleBondTail = 1, kRoleAngleHead = 2, kRoleAngleTail = 3,
                    kRoleAngleMid = 4;
+
+// A bead of a stationary type (cleared after each step its world advances).
+struct StationaryBead { uint32_t bead; int32_t world; };
 
 // Device buffers of one layout.
 struct DeviceState {
@@ -80,9 +90,11 @@ struct DeviceState {
     BeadBlock *bead_blocks;    // kBeadThreads beads per block
     BeadBlock *force_blocks;   // kForceBeads beads per block
     BondedBead *bonded;
+    BondBlock *bond_blocks;
     uint32_t *bead_terms;
     BondRec *bonds;
     AngleRec *angles;
+    StationaryBead *stationary_beads;
 };
 
 }  // namespace pdpd

@@ -14,7 +14,8 @@ struct StepLaunch {
     uint32_t n_worlds;
     uint32_t n_bead_blocks;
     uint32_t n_force_blocks;
-    uint32_t n_bonded;
+    uint32_t n_bond_blocks;
+    uint32_t n_stationary;   // beads of stationary types
     uint32_t force_grid;     // 0 = one workgroup per force block, else persistent grid size
 };
 

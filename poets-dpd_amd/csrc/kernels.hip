@@ -94,9 +94,10 @@ __global__ void k_debug_rng(uint32_t n, const uint64_t *__restrict__ th, const u
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
-// Exclusive scan of one value per thread across a 256-thread block (4 wave64).
+// Exclusive scan of one value per thread across a block of NW wave64s.
 // Returns the exclusive prefix; *total receives the block sum.
-__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *s_wsum, uint32_t *total)
+template <int NW>
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total)
 {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -110,7 +111,7 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *s_wsum,
     __syncthreads();
     uint32_t base = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NW; k++) {
         uint32_t s = s_wsum[k];
         if (k < wid) base += s;
         tot += s;
@@ -150,19 +151,27 @@ __global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int c
         const float dt = W->dt, hdt = W->hdt;
         const float fv[3] = {f.x, f.y, f.z};
         float wv[3] = {v.x, v.y, v.z};
+        if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) {
+            // stationary type: x stays; the pair force sees v(t) (the imported v on
+            // the first step, 0 after) -- naive_dpd_engine_half_merge.hpp:327-331
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const float Lf = W->Lf[d];
-            float vv = fmaf(fv[d], hdt, wv[d]);     // v(t)       = w + f dt/2
-            float vh = fmaf(fv[d], hdt, vv);        // v(t+dt/2)  = v + f dt/2
-            float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
-            xn += (xn < 0.0f) ? Lf : 0.0f;
-            xn -= (xn >= Lf) ? Lf : 0.0f;
-            x[d] = xn;
-            wv[d] = vh;
+            for (int d = 0; d < 3; d++) wv[d] = fmaf(fv[d], hdt, wv[d]);
+            S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float Lf = W->Lf[d];
+                float vv = fmaf(fv[d], hdt, wv[d]);     // v(t)       = w + f dt/2
+                float vh = fmaf(fv[d], hdt, vv);        // v(t+dt/2)  = v + f dt/2
+                float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
+                xn += (xn < 0.0f) ? Lf : 0.0f;
+                xn -= (xn >= Lf) ? Lf : 0.0f;
+                x[d] = xn;
+                wv[d] = vh;
+            }
+            S.pos[cur][s] = make_float4(x[0], x[1], x[2], p.w);
+            S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
         }
-        S.pos[cur][s] = make_float4(x[0], x[1], x[2], p.w);
-        S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
     }
     bool bad = false;
 #pragma unroll
@@ -187,7 +196,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ co
                                                       uint32_t *__restrict__ ticket, uint32_t n_tiles,
                                                       uint32_t *__restrict__ flags)
 {
-    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_wsum[kScanThreads / 64];
     __shared__ uint32_t s_tile, s_prefix;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ co
         acc += v[k];
     }
     uint32_t agg;
-    uint32_t ex = block_scan_256(acc, s_wsum, &agg);
+    uint32_t ex = block_scan<kScanThreads / 64>(acc, s_wsum, &agg);
     if (threadIdx.x == 0) {
         uint32_t prefix = 0;
         if (tile == 0) {
@@ -305,7 +314,9 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
         const uint32_t s = blk.begin + lane;
         const bool live = s < blk.end;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (live) {
+        // a stationary home bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390);
+        // its pairs with moving beads are evaluated from the moving side
+        if (live && !((W->stationary >> (fbits(pos[s].w) >> 28)) & 1u)) {
             const WorldCtr wc = S.ctr[blk.world];
             const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
             const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
@@ -317,44 +328,66 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
             const uint32_t trow = (hh >> 28) * kMaxTypes;
             const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
             uint32_t cnt = 0;
+            // one accepted pair: dpd_maths_core_half_step_raw.hpp:129-216 with the
+            // terms grouped as a wr + sgw (scale u - sgw rdotv)
+            auto pair = [&](uint32_t e, const float4 &op, const float4 &ov) {
+                const float ox = op.x + shift_of((e >> 26) & 3u, W->Lf[0]);
+                const float oy = op.y + shift_of((e >> 28) & 3u, W->Lf[1]);
+                const float oz = op.z + shift_of((e >> 30) & 3u, W->Lf[2]);
+                const float dx = hp.x - ox, dy = hp.y - oy, dz = hp.z - oz;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float rr = sqrtf(r2);
+                const float inv_r = 1.0f / rr;
+                const float wr = 1.0f - rr;
+                const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+                const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
+                const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+                const uint32_t oh = fbits(op.w);
+                const uint32_t ti = trow + (oh >> 28);
+                const float sgw = W->sg[ti] * wr;
+                const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
+                const float F = fmaf(W->a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                fx = fmaf(ex, F, fx);
+                fy = fmaf(ey, F, fy);
+                fz = fmaf(ez, F, fz);
+            };
+            // evaluate the buffered pairs in order, loads of two pairs in flight
             auto flush = [&]() {
-                for (uint32_t q = 0; q < cnt; q++) {
+                uint32_t q = 0;
+                for (; q + 2 <= cnt; q += 2) {
+                    const uint32_t e0 = s_list[q * kForceThreads + tid];
+                    const uint32_t e1 = s_list[(q + 1) * kForceThreads + tid];
+                    const float4 op0 = pos[e0 & kSlotMask], ov0 = vel[e0 & kSlotMask];
+                    const float4 op1 = pos[e1 & kSlotMask], ov1 = vel[e1 & kSlotMask];
+                    pair(e0, op0, ov0);
+                    pair(e1, op1, ov1);
+                }
+                if (q < cnt) {
                     const uint32_t e = s_list[q * kForceThreads + tid];
-                    const uint32_t m = e & kSlotMask;
-                    const float4 op = pos[m];
-                    const float4 ov = vel[m];
-                    const float ox = op.x + shift_of((e >> 26) & 3u, W->Lf[0]);
-                    const float oy = op.y + shift_of((e >> 28) & 3u, W->Lf[1]);
-                    const float oz = op.z + shift_of((e >> 30) & 3u, W->Lf[2]);
-                    const float dx = hp.x - ox, dy = hp.y - oy, dz = hp.z - oz;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float rr = sqrtf(r2);
-                    const float inv_r = 1.0f / rr;
-                    const float wr = 1.0f - rr;
-                    const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
-                    const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
-                    const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                    const uint32_t oh = fbits(op.w);
-                    const uint32_t ti = trow + (oh >> 28);
-                    const float sgw = W->sg[ti] * wr;
-                    const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
-                    const float F = fmaf(W->a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
-                    fx = fmaf(ex, F, fx);
-                    fy = fmaf(ey, F, fy);
-                    fz = fmaf(ez, F, fz);
+                    pair(e, pos[e & kSlotMask], vel[e & kSlotMask]);
                 }
                 cnt = 0;
             };
-            // candidates of cells [m0, m1) with one periodic image
+            auto test = [&](const float4 &op, uint32_t m, bool valid, uint32_t code, float sx, float sy, float sz) {
+                const float dx = hp.x - (op.x + sx), dy = hp.y - (op.y + sy), dz = hp.z - (op.z + sz);
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (valid && r2 < 1.0f && r2 >= kMinR2 && m != s) s_list[(cnt++) * kForceThreads + tid] = m | code;
+            };
+            // candidates of cells [m0, m1) with one periodic image, four loads in
+            // flight; the list is flushed between groups so no loaded candidate is
+            // live across a flush
             auto scan = [&](uint32_t m0, uint32_t m1, uint32_t code, float sx, float sy, float sz) {
-                for (uint32_t m = m0; m < m1; m++) {
-                    const float4 op = pos[m];
-                    const float dx = hp.x - (op.x + sx), dy = hp.y - (op.y + sy), dz = hp.z - (op.z + sz);
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    if (r2 < 1.0f && r2 >= kMinR2 && m != s) {
-                        s_list[cnt * kForceThreads + tid] = m | code;
-                        if (++cnt == kPairList) flush();
-                    }
+                for (uint32_t m = m0; m < m1; m += 4) {
+                    if (cnt > kPairList - 4) flush();
+                    const uint32_t last = m1 - 1;
+                    const float4 p0 = pos[m];
+                    const float4 p1 = pos[min(m + 1, last)];
+                    const float4 p2 = pos[min(m + 2, last)];
+                    const float4 p3 = pos[min(m + 3, last)];
+                    test(p0, m, true, code, sx, sy, sz);
+                    test(p1, m + 1, m + 1 < m1, code, sx, sy, sz);
+                    test(p2, m + 2, m + 2 < m1, code, sx, sy, sz);
+                    test(p3, m + 3, m + 3 < m1, code, sx, sy, sz);
                 }
             };
             int ox = cx + plane - 1;
@@ -396,89 +429,133 @@ __global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt,
 
 // -------------------------------------------------------------- bonds ----
 
-__device__ __forceinline__ float min_image(float d, float len)
+__device__ __forceinline__ double min_image(double d, double len)
 {
-    if (d > 0.5f * len) d -= len;
-    else if (d < -0.5f * len) d += len;
+    if (d > 0.5 * len) d -= len;
+    else if (d < -0.5 * len) d += len;
     return d;
 }
 
-// One thread per bonded bead: its Hookean and angle terms, each recomputed from
-// the positions of the term's beads exactly as the per-polymer loop computes
-// it, and added in that loop's order (bonds, then bond pairs; within a bond pair
-// head, tail, middle), so the sum equals the per-polymer sequence bit for bit.
-__global__ __launch_bounds__(kBeadThreads) void k_bonds(DeviceState S, int nxt, int par, uint32_t n_bonded,
+// Contribution of one bonded term to one of its beads, in FP64 from the FP32
+// positions (differences of floats are exact in double). FP64 keeps the
+// theta0 != 0 angle force, which divides by sin(phi), stable near straight
+// chains. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
+// dpd_maths_core_half_step.hpp:178-231 with the prefactor floor of
+// naive_dpd_engine.hpp:681-682 (the argument clamped at 0).
+__device__ void bond_term(const DeviceState &S, const float4 *__restrict__ pos, const WorldDev *__restrict__ W,
+                          uint32_t code, double &cx, double &cy, double &cz)
+{
+    const uint32_t role = code >> kRoleShift;
+    const uint32_t idx = code & ((1u << kRoleShift) - 1u);
+    const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+    if (role <= kRoleBondTail) {
+        const BondRec b = S.bonds[idx];
+        const float4 xh = pos[S.slot_of_bead[b.head]], xt = pos[S.slot_of_bead[b.tail]];
+        const double dx = min_image((double)xt.x - (double)xh.x, Lx);
+        const double dy = min_image((double)xt.y - (double)xh.y, Ly);
+        const double dz = min_image((double)xt.z - (double)xh.z, Lz);
+        const double r = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
+        const double hf = b.kappa * (b.r0 - r);
+        const double sc = hf / r;
+        const double sgn = role == kRoleBondHead ? -1.0 : 1.0;   // head -= f, tail += f
+        cx = sgn * (dx * sc);
+        cy = sgn * (dy * sc);
+        cz = sgn * (dz * sc);
+        return;
+    }
+    const AngleRec a = S.angles[idx];
+    const float4 xh = pos[S.slot_of_bead[a.head]], xm = pos[S.slot_of_bead[a.mid]], xt = pos[S.slot_of_bead[a.tail]];
+    const double d0x = min_image((double)xm.x - (double)xh.x, Lx), d0y = min_image((double)xm.y - (double)xh.y, Ly),
+                 d0z = min_image((double)xm.z - (double)xh.z, Lz);
+    const double d1x = min_image((double)xt.x - (double)xm.x, Lx), d1y = min_image((double)xt.y - (double)xm.y, Ly),
+                 d1z = min_image((double)xt.z - (double)xm.z, Lz);
+    const double r01 = sqrt(fma(d0z, d0z, fma(d0y, d0y, d0x * d0x)));
+    const double r12 = sqrt(fma(d1z, d1z, fma(d1y, d1y, d1x * d1x)));
+    const double mag = r01 * r12;
+    double hx = 0, hy = 0, hz = 0, tx = 0, ty = 0, tz = 0;
+    if (mag > 0.0001) {
+        const double b1 = r01 * r01, b2 = r12 * r12;
+        const double dot = fma(d0z, d1z, fma(d0y, d1y, d0x * d1x));
+        const double o1 = dot / b1, o2 = dot / b2;
+        const double cos2 = o1 * o2;
+        double fm = a.kappa / mag;
+        if (fabs(dot) > 0.000001 && a.sin0 > 0.0) {
+            const double pref = fmax(sqrt(fmax(1.0 / cos2 - 1.0, 0.0)), 0.000001);
+            fm = fm * (a.cos0 - a.sin0 * (1.0 / pref));
+        }
+        hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
+        tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
+    }
+    if (role == kRoleAngleHead) { cx = hx; cy = hy; cz = hz; }
+    else if (role == kRoleAngleTail) { cx = tx; cy = ty; cz = tz; }
+    else { cx = -(hx + tx); cy = -(hy + ty); cz = -(hz + tz); }
+}
+
+// Bonds per group of whole bonded beads: one lane per term computes its
+// contribution (all loads of a group in flight at once), then one lane per bead
+// adds its terms to the pair force in the per-polymer loop's order (bonds, then
+// bond pairs; within a bond pair head, tail, middle) in FP64 and rounds once.
+__global__ __launch_bounds__(kBondThreads) void k_bonds(DeviceState S, int nxt, int par, uint32_t n_blocks,
                                                        uint32_t n_worlds)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_c[3][kBondThreads];
+    const uint32_t tid = threadIdx.x;
     const WorldCtr *__restrict__ ctr_in = S.ctr + par * n_worlds;
-    if (i < n_bonded) {
-        const BondedBead bb = S.bonded[i];
-        if (ctr_in[bb.world].remaining > 0) {
-            const WorldDev *__restrict__ W = &S.worlds[bb.world];
-            const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+    if (blockIdx.x < n_blocks) {
+        const BondBlock bk = S.bond_blocks[blockIdx.x];
+        if (ctr_in[bk.world].remaining > 0) {                  // uniform per block
+            const WorldDev *__restrict__ W = &S.worlds[bk.world];
             const float4 *__restrict__ pos = S.pos[nxt];
-            const uint32_t me = S.slot_of_bead[bb.bead];
-            float4 f = S.frc[nxt][me];
-            for (uint32_t t = bb.t0; t < bb.t1; t++) {
-                const uint32_t code = S.bead_terms[t];
-                const uint32_t role = code >> kRoleShift;
-                const uint32_t idx = code & ((1u << kRoleShift) - 1u);
-                if (role <= kRoleBondTail) {
-                    const BondRec b = S.bonds[idx];
-                    const float4 xh = pos[S.slot_of_bead[b.head]], xt = pos[S.slot_of_bead[b.tail]];
-                    const float dx = min_image(xt.x - xh.x, Lx);
-                    const float dy = min_image(xt.y - xh.y, Ly);
-                    const float dz = min_image(xt.z - xh.z, Lz);
-                    const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-                    const float hf = b.kappa * (b.r0 - r);
-                    const float sc = hf / r;
-                    if (role == kRoleBondHead) { f.x -= dx * sc; f.y -= dy * sc; f.z -= dz * sc; }
-                    else { f.x += dx * sc; f.y += dy * sc; f.z += dz * sc; }
-                } else {
-                    const AngleRec a = S.angles[idx];
-                    const float4 xh = pos[S.slot_of_bead[a.head]], xm = pos[S.slot_of_bead[a.mid]],
-                                 xt = pos[S.slot_of_bead[a.tail]];
-                    const float d0x = min_image(xm.x - xh.x, Lx), d0y = min_image(xm.y - xh.y, Ly),
-                                d0z = min_image(xm.z - xh.z, Lz);
-                    const float d1x = min_image(xt.x - xm.x, Lx), d1y = min_image(xt.y - xm.y, Ly),
-                                d1z = min_image(xt.z - xm.z, Lz);
-                    const float r01 = sqrtf(fmaf(d0z, d0z, fmaf(d0y, d0y, d0x * d0x)));
-                    const float r12 = sqrtf(fmaf(d1z, d1z, fmaf(d1y, d1y, d1x * d1x)));
-                    const float mag = r01 * r12;
-                    float hx = 0.f, hy = 0.f, hz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
-                    if (mag > 0.0001f) {
-                        const float b1 = r01 * r01, b2 = r12 * r12;
-                        const float dot = fmaf(d0z, d1z, fmaf(d0y, d1y, d0x * d1x));
-                        const float o1 = dot / b1, o2 = dot / b2;
-                        const float cos2 = o1 * o2;
-                        float fm = a.kappa / mag;
-                        if (fabsf(dot) > 0.000001f && a.sin0 > 0.0f) {
-                            // |cot phi| with the naive engine's floor on the prefactor
-                            // (naive_dpd_engine.hpp:681-682); the argument is clamped at 0
-                            // because FP32 rounding can push cos^2 phi just above 1
-                            const float pref = fmaxf(sqrtf(fmaxf(1.0f / cos2 - 1.0f, 0.0f)), 0.000001f);
-                            const float ip = 1.0f / pref;
-                            fm = fm * (a.cos0 - a.sin0 * ip);
-                        }
-                        hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
-                        tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
-                    }
-                    if (role == kRoleAngleHead) { f.x += hx; f.y += hy; f.z += hz; }
-                    else if (role == kRoleAngleTail) { f.x += tx; f.y += ty; f.z += tz; }
-                    else { f.x += -(hx + tx); f.y += -(hy + ty); f.z += -(hz + tz); }
-                }
+            const uint32_t t = bk.term_begin + tid;
+            if (t < bk.term_end) {
+                double cx, cy, cz;
+                bond_term(S, pos, W, S.bead_terms[t], cx, cy, cz);
+                s_c[0][tid] = cx;
+                s_c[1][tid] = cy;
+                s_c[2][tid] = cz;
             }
-            S.frc[nxt][me] = f;
+            __syncthreads();
+            const uint32_t b = bk.bead_begin + tid;
+            if (b < bk.bead_end) {
+                const BondedBead bb = S.bonded[b];
+                const uint32_t me = S.slot_of_bead[bb.bead];
+                float4 f = S.frc[nxt][me];
+                double ax = f.x, ay = f.y, az = f.z;
+                for (uint32_t k = bb.t0 - bk.term_begin; k < bb.t1 - bk.term_begin; k++) {
+                    ax += s_c[0][k];
+                    ay += s_c[1][k];
+                    az += s_c[2][k];
+                }
+                f.x = (float)ax;
+                f.y = (float)ay;
+                f.z = (float)az;
+                if ((S.worlds[bk.world].stationary >> (fbits(pos[me].w) >> 28)) & 1u) f = make_float4(0, 0, 0, 0);
+                S.frc[nxt][me] = f;
+            }
         }
     }
     // step counters: read ctr[par], write ctr[par^1] (nothing reads that copy this step)
+    const uint32_t i = blockIdx.x * blockDim.x + tid;
     if (i < n_worlds) {
         const WorldCtr c = S.ctr[par * n_worlds + i];
         WorldCtr o = c;
         if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
         S.ctr[(par ^ 1) * n_worlds + i] = o;
     }
+}
+
+// After the pair forces have read them: beads of stationary types end the step
+// with v = 0 (naive_dpd_engine_half_merge.hpp:387-390). Runs only when such
+// beads are resident.
+__global__ __launch_bounds__(kBeadThreads) void k_stationary(DeviceState S, int nxt, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const StationaryBead e = S.stationary_beads[i];
+    if (S.ctr[e.world].remaining == 0) return;       // S.ctr points at this step's counters
+    const uint32_t slot = S.slot_of_bead[e.bead];
+    const float4 v = S.vel[nxt][slot];
+    S.vel[nxt][slot] = make_float4(0.0f, 0.0f, 0.0f, v.w);
 }
 
 // ------------------------------------------------------------ utility ----
@@ -529,8 +606,10 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     if (fg) k_force<<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks);
     rec(5);
-    const uint32_t pw = std::max<uint32_t>(L.n_bonded, L.n_worlds);
-    k_bonds<<<ceil_div(pw, kBeadThreads), kBeadThreads, 0, st>>>(S, cur ^ 1, par, L.n_bonded, L.n_worlds);
+    if (L.n_stationary)
+        k_stationary<<<ceil_div(L.n_stationary, kBeadThreads), kBeadThreads, 0, st>>>(Sp, cur ^ 1, L.n_stationary);
+    const uint32_t bg = std::max<uint32_t>(L.n_bond_blocks, ceil_div(L.n_worlds, kBondThreads));
+    k_bonds<<<bg, kBondThreads, 0, st>>>(S, cur ^ 1, par, L.n_bond_blocks, L.n_worlds);
     rec(6);
     return hipGetLastError();
 }

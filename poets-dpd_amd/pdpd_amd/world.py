@@ -196,11 +196,12 @@ def _gaussian_velocities(rng, n):
 
 
 def make_world(L, density=3.0, bonded_fraction=0.0, seed=1, a_matrix=None, gamma=4.5, dt=0.01,
-               n_beads=None):
-    """Water (type W) plus H2T4 chains with Hookean k=128, r0=0.5 and straight
-    angle bonds k=20. Positions uniform from the seed, velocities Gaussian at
-    kT=1 with zero net momentum, forces zero. Chains are numbered first
-    (polymer ids below 2^20, as BeadHash requires), then the monomers."""
+               n_beads=None, theta0=0.0):
+    """Water (type W) plus H2T4 chains with Hookean k=128, r0=0.5 and angle
+    bonds k=20 with preferred angle theta0 (0 = straight). Positions uniform from
+    the seed, velocities Gaussian at kT=1 with zero net momentum, forces zero.
+    Chains are numbered first (polymer ids below 2^20, as BeadHash requires),
+    then the monomers."""
     rng = np.random.default_rng(seed)
     box = np.asarray(L, dtype=np.float64)
     n = int(round(density * float(np.prod(box)))) if n_beads is None else int(n_beads)
@@ -218,7 +219,8 @@ def make_world(L, density=3.0, bonded_fraction=0.0, seed=1, a_matrix=None, gamma
     diss = np.full((nt, nt), float(gamma))
     ptypes = [PolymerType("W", [0])]
     if n_chains:
-        ptypes.append(PolymerType("HHTTTT", [1, 1, 2, 2, 2, 2], list(CHAIN_BONDS), list(CHAIN_ANGLES)))
+        angles = [(h, t, k, float(theta0)) for h, t, k, _ in CHAIN_ANGLES]
+        ptypes.append(PolymerType("HHTTTT", [1, 1, 2, 2, 2, 2], list(CHAIN_BONDS), angles))
     n_mono = n - n_chains * clen
     v = _gaussian_velocities(rng, n)
     # chains: random walks of step 0.5 from a uniform start, wrapped
@@ -252,6 +254,32 @@ def make_world(L, density=3.0, bonded_fraction=0.0, seed=1, a_matrix=None, gamma
                  polymer_bead_start=starts, polymer_bead_ids=np.arange(n, dtype=np.uint32),
                  x=x, v=v, f=np.zeros((n, 3)), bead_type=bead_type, is_monomer=is_mono,
                  polymer_id=pid, polymer_offset=poff)
+
+
+def add_wall(w, thickness=1.0, a_wall=25.0, gamma_wall=4.5):
+    """Turn the monomers with x < thickness into a new stationary bead type "S"
+    (BeadType::stationary; a wall): velocities set to zero, interactions of S
+    with every type a_wall / gamma_wall. Returns a new World."""
+    w = w.copy()
+    nt = w.n_types
+    sel = (w.is_monomer == 1) & (w.x[:, 0] < thickness)
+    cons = np.full((nt + 1, nt + 1), float(a_wall))
+    diss = np.full((nt + 1, nt + 1), float(gamma_wall))
+    cons[:nt, :nt] = w.conservative
+    diss[:nt, :nt] = w.dissipative
+    w.conservative, w.dissipative = cons, diss
+    w.bead_type_names = list(w.bead_type_names) + ["S"]
+    w.stationary = np.zeros(nt + 1, np.uint8)
+    w.stationary[nt] = 1
+    w.polymer_types = list(w.polymer_types) + [PolymerType("S", [nt])]
+    w.bead_type = w.bead_type.copy()
+    w.bead_type[sel] = nt
+    ptype_of = w.polymer_type_of.copy()
+    ptype_of[w.polymer_id[sel]] = len(w.polymer_types) - 1
+    w.polymer_type_of = ptype_of
+    w.v = w.v.copy()
+    w.v[sel] = 0.0
+    return w
 
 
 def random_a_matrix(rng, choices=(25.0, 35.0, 50.0, 75.0), nt=3):

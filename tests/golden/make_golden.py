@@ -1,7 +1,7 @@
 """Write the golden fixtures of tests/golden/ from the reference's own code.
 
 Run in the build container, where the reference tree exists:
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [case ...]
 The reference is compiled from its headers by oracle/Makefile into
 oracle/_ref/libref_shim.so (hash.hpp, dpd_maths_core.hpp, dpd_state.hpp,
 NaiveDPDEngine<false>, NaiveDPDEngineHalfMerge). This script only calls that
@@ -72,6 +72,10 @@ def naive_cases():
         "poly8": W.make_world((8, 8, 8), bonded_fraction=0.1, seed=13, a_matrix=W.random_a_matrix(rngA)),
         # dense start: 8 beads per cell in a 2x2x2 lattice of one region
         "mesh4": _mesh_world(),
+        # stationary wall slab (BeadType::stationary) and chains with a preferred
+        # angle theta0 = 1 rad (the sin(theta0) / prefactor branch of the angle force)
+        "wall_bent8": W.add_wall(W.make_world((8, 8, 8), bonded_fraction=0.2, seed=14, theta0=1.0,
+                                              a_matrix=W.random_a_matrix(rngA))),
     }
 
 
@@ -91,8 +95,10 @@ def _mesh_world():
     return b.build()
 
 
-def naive_fixtures():
+def naive_fixtures(only=None):
     for name, w in naive_cases().items():
+        if only and name not in only:
+            continue
         out = world_to_arrays(w)
         for steps in (1, 10):
             x, v, f, t = O.ref_run(w, "naive", steps)
@@ -109,8 +115,10 @@ def naive_fixtures():
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("build the reference shim first: make -C oracle ref")
-    rng_fixture()
-    naive_fixtures()
+    only = sys.argv[1:]            # fixture names to (re)write; default all
+    if not only:
+        rng_fixture()
+    naive_fixtures(only)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

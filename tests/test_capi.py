@@ -45,9 +45,11 @@ def test_abi_version_and_limits():
 def test_can_support_reasons():
     w = W.make_world((4, 4, 4), seed=1)
     assert capi.can_support(w) == ""
+    assert capi.can_support(W.add_wall(w)) == ""          # stationary bead types are supported
     w2 = w.copy()
-    w2.stationary = np.array([1], np.uint8)
-    assert capi.can_support(w2) == "Engine does not support stationary bead types."
+    w2.conservative = np.full((17, 17), 25.0)
+    w2.dissipative = np.full((17, 17), 4.5)
+    assert capi.can_support(w2) == "Engine supports 1 to 16 bead types."
     w3 = w.copy()
     w3.box = np.array([4.5, 4, 4])
     assert capi.can_support(w3) == "World bounds must be integer."

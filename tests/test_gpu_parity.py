@@ -46,6 +46,9 @@ def small_worlds():
         "poly_6x7x8": W.make_world((6, 7, 8), bonded_fraction=0.2, seed=12, a_matrix=W.random_a_matrix(rngA)),
         "poly_9x5x13": W.make_world((9, 5, 13), bonded_fraction=0.15, seed=14, a_matrix=W.random_a_matrix(rngA)),
         "water16": W.config_water(seed=3),
+        # stationary wall slab + chains with theta0 = 1 rad
+        "wall_bent8": W.add_wall(W.make_world((8, 8, 8), bonded_fraction=0.2, seed=15, theta0=1.0,
+                                              a_matrix=W.random_a_matrix(rngA))),
     }
 
 
@@ -85,7 +88,7 @@ def test_one_step_vs_oracle_f32(gpu_lib, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["poly_6x7x8", "water16"])
+@pytest.mark.parametrize("name", ["poly_6x7x8", "water16", "wall_bent8"])
 def test_multi_step_vs_oracle_f32(gpu_lib, name):
     w = small_worlds()[name]
     _say(f"20 steps: {name}")
@@ -225,10 +228,29 @@ def test_dense_cell_matches_oracle(gpu_lib):
     assert np.array_equal(f.view(np.uint32), ref.f.view(np.uint32))
 
 
+def test_stationary_beads_stay_put(gpu_lib):
+    """Beads of a stationary type keep x and end every step with v = f = 0
+    (naive_dpd_engine_half_merge.hpp:327-331,387-390); their neighbours still
+    feel them."""
+    w = W.add_wall(W.make_world((6, 6, 6), seed=21))
+    sel = w.bead_type == w.n_types - 1
+    _say(f"stationary wall: {int(sel.sum())} of {w.n_beads} beads")
+    ctx, slot = run_gpu(w, 7)
+    x, v, f, t = ctx.download(slot)
+    assert np.array_equal(x[sel], w.x[sel].astype(np.float32).astype(np.float64))
+    assert np.all(v[sel] == 0) and np.all(f[sel] == 0)
+    assert np.abs(f[~sel]).max() > 0
+    ref = O.F32State(w).step(7)
+    x32, w32, f32, _ = ctx.download_f32(slot)
+    assert np.array_equal(x32.view(np.uint32), ref.x.view(np.uint32))
+    assert np.array_equal(f32.view(np.uint32), ref.f.view(np.uint32))
+    ctx.close()
+
+
 def test_unsupported_world_reason(gpu_lib):
     w = W.make_world((4, 4, 4), seed=1)
-    w.stationary = np.array([1], np.uint8)
-    assert capi.can_support(w) == "Engine does not support stationary bead types."
+    w.box = np.array([4.5, 4.0, 4.0])
+    assert capi.can_support(w) == "World bounds must be integer."
     ctx = capi.Context(0)
     with pytest.raises(capi.PdpdError) as e:
         ctx.add(w)

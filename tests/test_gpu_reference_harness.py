@@ -21,28 +21,55 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_engine_tests")
 
 
-def _run(mode):
+FAMILIES = ["TestIsolatedMovingBead", "TestMultipleMovingBeads", "TestCollidingBeadPair", "TestCollidingBeadPairDiss",
+            "TestBondedBeadPair", "TestBondedBeadTriple", "TestBondedBeadAngle", "TestOrderedMesh"]
+MIN_TESTS = {"scenarios": 200, "differential": 200}      # the registered suite holds 225 scenarios
+
+
+def _run(mode, family):
+    """Run one scenario family (name prefix up to '['), streaming each TAP line as
+    it arrives so a long family shows progress."""
     if not os.path.exists(BIN):
         pytest.skip("oracle/_ref/ref_engine_tests not built (needs the reference tree at build time)")
-    print(f"reference harness: {mode}", flush=True)
-    p = subprocess.run([BIN, mode], capture_output=True, text=True, timeout=600)
-    lines = [l for l in p.stdout.splitlines() if l.startswith("ok ") or l.startswith("not ok ")]
+    print(f"reference harness: {mode} {family}", flush=True)
+    p = subprocess.Popen([BIN, mode, family + "["], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    lines = []
+    for line in p.stdout:
+        if line.startswith("ok ") or line.startswith("not ok "):
+            lines.append(line.rstrip())
+            print("  " + line.rstrip()[:160], flush=True)
+    err = p.stderr.read()
+    rc = p.wait(timeout=60)
     failed = [l for l in lines if l.startswith("not ok")]
     print(f"  {len(lines)} tests, {len(failed)} failed", flush=True)
-    for l in failed[:20]:
-        print("  " + l, flush=True)
-    return p.returncode, lines, failed, p.stderr
+    return rc, lines, failed, err
 
 
-def test_reference_scenarios_pass_on_hip_engine():
-    rc, lines, failed, err = _run("scenarios")
-    assert len(lines) >= 200
+_counts = {}
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+def test_reference_scenarios_pass_on_hip_engine(family):
+    rc, lines, failed, err = _run("scenarios", family)
+    _counts[("scenarios", family)] = len(lines)
+    assert lines, f"no {family} scenarios ran"
     assert not failed, "\n".join(failed[:10]) + "\n" + err[-2000:]
     assert rc == 0
 
 
-def test_reference_differential_vs_naive_engine():
-    rc, lines, failed, err = _run("differential")
-    assert len(lines) >= 200
+@pytest.mark.parametrize("family", FAMILIES)
+def test_reference_differential_vs_naive_engine(family):
+    rc, lines, failed, err = _run("differential", family)
+    _counts[("differential", family)] = len(lines)
+    assert lines, f"no {family} scenarios ran"
     assert not failed, "\n".join(failed[:10]) + "\n" + err[-2000:]
     assert rc == 0
+
+
+def test_reference_suite_complete():
+    """Every registered scenario ran in both modes (the families cover the suite)."""
+    for mode, need in MIN_TESTS.items():
+        got = sum(v for (m, _), v in _counts.items() if m == mode)
+        if not any(m == mode for (m, _) in _counts):
+            pytest.skip("families not run in this session")
+        assert got >= need, f"{mode}: {got} scenarios"
