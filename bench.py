@@ -190,7 +190,7 @@ def main():
     st = ctx.stats()
     force_ms = st["k_force_ms"]
     step_ms = sum(st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms", "k_reorder_ms",
-                                  "k_force_ms", "k_bonds_ms"))
+                                  "k_force_ms", "k_stationary_ms"))
     achieved = n * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -199,7 +199,7 @@ def main():
                 "valu_fp32_frac": n * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
                 "step_hbm_frac": n * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
                 "kernel_ms_per_step": {k[2:-3]: st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms",
-                                                               "k_reorder_ms", "k_force_ms", "k_bonds_ms")}}
+                                                               "k_reorder_ms", "k_force_ms", "k_stationary_ms")}}
     x, v, f, t = ctx.download(slot)
     ctx.close()
 

@@ -160,8 +160,9 @@ typedef struct pdpd_stats {
     uint32_t n_worlds;
     uint64_t n_beads;
     uint32_t n_force_blocks;      /* work items of the pair-force kernel (64 beads each)          */
-    /* average kernel durations (ms) of the last profiled run, 0 if none */
-    double k_integrate_ms, k_scan_ms, k_scatter_ms, k_reorder_ms, k_force_ms, k_bonds_ms;
+    /* average kernel durations (ms) of the last profiled run, 0 if none; k_force includes the
+     * bonded terms, k_stationary the velocity clear of stationary beads (0 without them) */
+    double k_integrate_ms, k_scan_ms, k_scatter_ms, k_reorder_ms, k_force_ms, k_stationary_ms;
     uint64_t profiled_steps;
 } pdpd_stats;
 

@@ -447,11 +447,13 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
         const float hvx = w->w[3 * i], hvy = w->w[3 * i + 1], hvz = w->w[3 * i + 2];
         const uint32_t hh = w->hash[i];
         const int htype = (int)(hh >> 28);
-        /* one partial sum per x-plane of the stencil, combined (p0 + p1) + p2 */
+        /* one partial sum per (x-plane, y-row) of the stencil, summed over the row's
+         * cells in z order; rows combined (r0 + r1) + r2 per plane, planes (p0 + p1) + p2 */
         float px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, pz[3] = {0, 0, 0}, fa = 0.0f;
         for (int ddx = -1; ddx <= 1; ddx++) {
+        float rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0};
+        for (int ddy = -1; ddy <= 1; ddy++) {
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (int ddy = -1; ddy <= 1; ddy++)
         for (int ddz = -1; ddz <= 1; ddz++) {
             int oc3[3];
             float sh[3];
@@ -490,9 +492,13 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
                 fa += fabsf(F);
             }
         }
-        px[ddx + 1] = fx;
-        py[ddx + 1] = fy;
-        pz[ddx + 1] = fz;
+        rx[ddy + 1] = fx;
+        ry[ddy + 1] = fy;
+        rz[ddy + 1] = fz;
+        }
+        px[ddx + 1] = (rx[0] + rx[1]) + rx[2];
+        py[ddx + 1] = (ry[0] + ry[1]) + ry[2];
+        pz[ddx + 1] = (rz[0] + rz[1]) + rz[2];
         }
         f_out[3 * i] = (px[0] + px[1]) + px[2];
         f_out[3 * i + 1] = (py[0] + py[1]) + py[2];
@@ -522,10 +528,10 @@ static double norm3_f64(const double *v)
     return sqrt(fma(v[2], v[2], fma(v[1], v[1], v[0] * v[0])));
 }
 
-/* Bonded terms in FP64 from the FP32 positions. Each bead's terms are added to
- * its FP32 pair force in a double accumulator, in the order the per-polymer loop
- * applies them (the polymer's bonds, then its bond pairs), and rounded to float
- * once. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
+/* Bonded terms in FP64 from the FP32 positions. Each bead's terms are summed in
+ * a double accumulator starting from 0, in the order the per-polymer loop
+ * applies them (the polymer's bonds, then its bond pairs); a bead with terms
+ * gets f = (float)((double)f_pair + sum), rounded once. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
  * dpd_maths_core_half_step.hpp:178-231 via naive_dpd_engine_half_merge.hpp:656-690,
  * with the prefactor floor of naive_dpd_engine.hpp:681-682 (argument clamped at 0). */
 void orc_f32_bonds(const orc_f32_world *w, float *F)
@@ -533,7 +539,7 @@ void orc_f32_bonds(const orc_f32_world *w, float *F)
     int n = w->n_beads;
     double *acc = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
     unsigned char *touched = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
-    for (int i = 0; i < 3 * n; i++) acc[i] = (double)F[i];
+    for (int i = 0; i < 3 * n; i++) acc[i] = 0.0;
     for (int p = 0; p < w->n_polymers; p++) {
         for (int bi = w->poly_bond_start[p]; bi < w->poly_bond_start[p + 1]; bi++) {
             int h = w->bond_head[bi], t = w->bond_tail[bi];
@@ -581,7 +587,7 @@ void orc_f32_bonds(const orc_f32_world *w, float *F)
     }
     for (int i = 0; i < n; i++)
         if (touched[i] && !f32_stationary(w, i))
-            for (int d = 0; d < 3; d++) F[3 * i + d] = (float)acc[3 * i + d];
+            for (int d = 0; d < 3; d++) F[3 * i + d] = (float)((double)F[3 * i + d] + acc[3 * i + d]);
     free(acc);
     free(touched);
 }

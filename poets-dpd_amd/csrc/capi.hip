@@ -82,8 +82,7 @@ struct WorldHost {
     float a[kMaxTypes * kMaxTypes] = {};
     float sg[kMaxTypes * kMaxTypes] = {};
     std::vector<float4> up_pos, up_vel, up_frc;   // pending upload (bead order)
-    std::vector<BondRec> bonds;                    // local bead indices, reference order
-    std::vector<AngleRec> angles;
+    std::vector<TermRec> bonds, angles;            // local bead indices, reference order
     std::vector<uint32_t> stationary_beads;        // local indices of beads of stationary types
     uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
@@ -138,20 +137,18 @@ struct pdpd_ctx {
     DevBuf<WorldDev> wdev;
     DevBuf<WorldCtr> ctr;
     DevBuf<BeadBlock> bblocks, fblocks;
-    DevBuf<BondedBead> bonded;
-    DevBuf<BondBlock> bond_blocks;
+    DevBuf<uint32_t> term_start, bead_terms;
+    DevBuf<TermRec> terms;
     DevBuf<StationaryBead> stationary;
-    DevBuf<uint32_t> bead_terms;
-    DevBuf<BondRec> bonds;
-    DevBuf<AngleRec> angles;
     int cur = 0;
-    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_bond_blocks = 0, n_stationary = 0;
+    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0;
     std::vector<int> order;               // slots in layout order
     std::vector<WorldCtr> ctr_host;
 
     // configuration
     uint32_t graph_steps = 20;
     uint32_t force_grid = 0;
+    uint32_t force_variant = 0;
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -167,8 +164,8 @@ struct pdpd_ctx {
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
         key.release(); rank.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
-        wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); bonded.release(); bond_blocks.release();
-        stationary.release(); bead_terms.release(); bonds.release(); angles.release();
+        wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
+        bead_terms.release(); terms.release(); stationary.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -181,8 +178,8 @@ struct pdpd_ctx {
         S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
         S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
         S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
-        S.bonded = bonded.p; S.bond_blocks = bond_blocks.p; S.bead_terms = bead_terms.p;
-        S.bonds = bonds.p; S.angles = angles.p; S.stationary_beads = stationary.p;
+        S.term_start = term_start.p; S.bead_terms = bead_terms.p; S.terms = terms.p;
+        S.stationary_beads = stationary.p;
         return S;
     }
 
@@ -192,44 +189,28 @@ struct pdpd_ctx {
         L.S = state();
         L.cur = c;
         L.n_beads = n_beads; L.n_cells = n_cells; L.n_worlds = n_worlds;
-        L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks; L.n_bond_blocks = n_bond_blocks;
+        L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks;
         L.n_stationary = n_stationary;
         L.force_grid = force_grid;
+        L.force_variant = force_variant;
         return L;
     }
 };
 
 namespace {
 
-// Work lists over all resident worlds. Force blocks of large worlds are
-// interleaved with those of small ones so a persistent grid sees an even mix.
+// Work lists over all resident worlds, in layout (world, then cell) order. The
+// pair-force kernel deals contiguous shares of this list to the XCDs.
 void build_blocks(const std::vector<const WorldHost *> &ws, std::vector<BeadBlock> &bb, std::vector<BeadBlock> &fb)
 {
     bb.clear();
     fb.clear();
-    std::vector<std::vector<BeadBlock>> per(ws.size());
     for (size_t i = 0; i < ws.size(); i++) {
         const WorldHost &w = *ws[i];
         for (uint32_t b = 0; b < w.n; b += kBeadThreads)
             bb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kBeadThreads), 0});
         for (uint32_t b = 0; b < w.n; b += kForceBeads)
-            per[i].push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kForceBeads), 0});
-    }
-    // round-robin by world, largest first
-    std::vector<size_t> idx(ws.size());
-    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return per[a].size() > per[b].size(); });
-    size_t round = 0;
-    bool any = true;
-    while (any) {
-        any = false;
-        for (size_t i : idx) {
-            if (round < per[i].size()) {
-                fb.push_back(per[i][round]);
-                any = true;
-            }
-        }
-        round++;
+            fb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kForceBeads), 0});
     }
 }
 
@@ -351,62 +332,32 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_upload(ctx->bblocks, bb));
     HIP_TRY(dev_upload(ctx->fblocks, fb));
 
-    // bonded topology with global bead indices, and per bonded bead its terms in
-    // the order the per-polymer loop applies them
-    std::vector<BondRec> brec;
-    std::vector<AngleRec> arec;
-    std::vector<BondedBead> bbead;
-    std::vector<BondBlock> bblk;
-    std::vector<uint32_t> terms;
+    // bonded terms with global bead indices; per bead (global index) its terms
+    // in the order the per-polymer loop applies them: its bonds, then its angles
+    std::vector<TermRec> trec;
+    std::vector<uint32_t> tstart(nb + 1, 0), terms;
     for (size_t i = 0; i < ws.size(); i++) {
         const WorldHost &w = *ws[i];
         if (w.bonds.empty() && w.angles.empty()) continue;
         std::vector<std::vector<uint32_t>> per(w.n);
-        const uint32_t b0 = (uint32_t)brec.size(), a0 = (uint32_t)arec.size();
-        for (size_t k = 0; k < w.bonds.size(); k++) {
-            BondRec r = w.bonds[k];
-            per[r.head].push_back((kRoleBondHead << kRoleShift) | (b0 + (uint32_t)k));
-            per[r.tail].push_back((kRoleBondTail << kRoleShift) | (b0 + (uint32_t)k));
-            r.head += w.bead_off; r.tail += w.bead_off;
-            brec.push_back(r);
-        }
-        for (size_t k = 0; k < w.angles.size(); k++) {
-            AngleRec r = w.angles[k];
-            per[r.head].push_back((kRoleAngleHead << kRoleShift) | (a0 + (uint32_t)k));
-            per[r.tail].push_back((kRoleAngleTail << kRoleShift) | (a0 + (uint32_t)k));
-            per[r.mid].push_back((kRoleAngleMid << kRoleShift) | (a0 + (uint32_t)k));
-            r.head += w.bead_off; r.mid += w.bead_off; r.tail += w.bead_off;
-            arec.push_back(r);
-        }
-        // groups of whole bonded beads of this world, at most kBondThreads terms each
-        bool open = false;
-        for (uint32_t b = 0; b < w.n; b++) {
-            if (per[b].empty()) continue;
-            if (per[b].size() > (size_t)kBondThreads)
-                return fail(PDPD_ERR_INVALID, "a bead takes part in more than " + std::to_string(kBondThreads) +
-                                                  " bonds and angles");
-            if (!open || terms.size() + per[b].size() - bblk.back().term_begin > (size_t)kBondThreads) {
-                BondBlock k{};
-                k.bead_begin = k.bead_end = (uint32_t)bbead.size();
-                k.term_begin = k.term_end = (uint32_t)terms.size();
-                k.world = (int32_t)i;
-                bblk.push_back(k);
-                open = true;
-            }
-            BondedBead e{w.bead_off + b, (uint32_t)terms.size(), 0, (int32_t)i};
-            terms.insert(terms.end(), per[b].begin(), per[b].end());
-            e.t1 = (uint32_t)terms.size();
-            bbead.push_back(e);
-            bblk.back().bead_end = (uint32_t)bbead.size();
-            bblk.back().term_end = (uint32_t)terms.size();
-        }
+        auto add = [&](const TermRec &r0, std::initializer_list<std::pair<uint32_t, uint32_t>> roles) {
+            const uint32_t k = (uint32_t)trec.size();
+            for (const auto &rb : roles) per[rb.second].push_back((rb.first << kRoleShift) | k);
+            TermRec r = r0;
+            r.a += w.bead_off; r.b += w.bead_off; r.c += w.bead_off;
+            trec.push_back(r);
+        };
+        for (const TermRec &r : w.bonds) add(r, {{kRoleBondHead, r.a}, {kRoleBondTail, r.b}});
+        for (const TermRec &r : w.angles) add(r, {{kRoleAngleHead, r.a}, {kRoleAngleTail, r.c}, {kRoleAngleMid, r.b}});
+        if (trec.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
+        for (uint32_t b = 0; b < w.n; b++) tstart[w.bead_off + b + 1] = (uint32_t)per[b].size();
+        for (uint32_t b = 0; b < w.n; b++) terms.insert(terms.end(), per[b].begin(), per[b].end());
     }
-    if (brec.size() + arec.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
-    ctx->n_bond_blocks = (uint32_t)bblk.size();
-    HIP_TRY(dev_upload(ctx->bonds, brec));
-    HIP_TRY(dev_upload(ctx->angles, arec));
-    HIP_TRY(dev_upload(ctx->bonded, bbead));
-    HIP_TRY(dev_upload(ctx->bond_blocks, bblk));
+    // counts -> starts (beads of worlds without terms count 0); terms were appended in bead order
+    for (size_t k = 0; k < nb; k++) tstart[k + 1] += tstart[k];
+    HIP_TRY(dev_upload(ctx->terms, trec));
+    HIP_TRY(dev_upload(ctx->term_start, tstart));
+    HIP_TRY(dev_upload(ctx->bead_terms, terms));
 
     std::vector<StationaryBead> sb;
     for (size_t i = 0; i < ws.size(); i++)
@@ -545,22 +496,6 @@ std::string support_reason(const pdpd_world_desc *w)
     if (!(w->dt > 0)) return "dt must be positive.";
     if (w->n_bead_types == 0 || w->n_bead_types > (uint32_t)kMaxTypes)
         return "Engine supports 1 to " + std::to_string(kMaxTypes) + " bead types.";
-    // bonded terms per bead (per polymer-type offset): the bond kernel gives each
-    // term of a bead a lane of one workgroup
-    for (uint32_t pt = 0; pt < w->n_polymer_types; pt++) {
-        uint32_t cnt[kMaxPolymerLength + 1] = {};
-        const uint32_t b0 = w->pt_bond_start[pt], b1 = w->pt_bond_start[pt + 1];
-        auto add = [&](uint32_t off) { if (off <= kMaxPolymerLength) cnt[off]++; };
-        for (uint32_t b = b0; b < b1; b++) { add(w->bond_head[b]); add(w->bond_tail[b]); }
-        for (uint32_t a = w->pt_angle_start[pt]; a < w->pt_angle_start[pt + 1]; a++) {
-            const uint32_t hb = b0 + w->angle_head_bond[a], tb = b0 + w->angle_tail_bond[a];
-            if (hb >= b1 || tb >= b1) continue;      // rejected with a message at import
-            add(w->bond_head[hb]); add(w->bond_tail[hb]); add(w->bond_tail[tb]);
-        }
-        for (uint32_t k = 0; k <= kMaxPolymerLength; k++)
-            if (cnt[k] > (uint32_t)kBondThreads)
-                return "A bead takes part in more than " + std::to_string(kBondThreads) + " bonds and angles.";
-    }
     for (uint32_t p = 0; p < w->n_polymers; p++) {
         uint32_t len = w->polymer_bead_start[p + 1] - w->polymer_bead_start[p];
         if (len > kMaxPolymerLength) return "Polymer longer than 127 beads.";
@@ -600,7 +535,7 @@ int pdpd_get_limits(pdpd_limits *out)
     out->max_box = kMaxBox;
     out->max_polymer_length = kMaxPolymerLength;
     out->max_beads = kMaxLayoutBeads - 1;
-    out->max_bonds_per_bead = kBondThreads;
+    out->max_bonds_per_bead = 0xffffffffu;
     return PDPD_OK;
 }
 
@@ -692,27 +627,34 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
         const uint32_t len = d->polymer_bead_start[p + 1] - d->polymer_bead_start[p];
         for (uint32_t b = d->pt_bond_start[pt]; b < d->pt_bond_start[pt + 1]; b++) {
             if (d->bond_head[b] >= len || d->bond_tail[b] >= len) return fail(PDPD_ERR_INVALID, "bond offset out of range");
-            w.bonds.push_back({ids[d->bond_head[b]], ids[d->bond_tail[b]], d->bond_kappa[b], d->bond_r0[b]});
+            TermRec r{};
+            r.a = ids[d->bond_head[b]];
+            r.b = r.c = ids[d->bond_tail[b]];
+            r.kind = kTermBond;
+            r.kappa = d->bond_kappa[b];
+            r.p0 = d->bond_r0[b];
+            w.bonds.push_back(r);
         }
         for (uint32_t a = d->pt_angle_start[pt]; a < d->pt_angle_start[pt + 1]; a++) {
             const uint32_t hb = d->pt_bond_start[pt] + d->angle_head_bond[a];
             const uint32_t tb = d->pt_bond_start[pt] + d->angle_tail_bond[a];
             if (hb >= d->pt_bond_start[pt + 1] || tb >= d->pt_bond_start[pt + 1])
                 return fail(PDPD_ERR_INVALID, "bond pair offset out of range");
-            AngleRec r{};
-            r.head = ids[d->bond_head[hb]];
-            r.mid = ids[d->bond_tail[hb]];
-            r.tail = ids[d->bond_tail[tb]];
+            TermRec r{};
+            r.a = ids[d->bond_head[hb]];        // head
+            r.b = ids[d->bond_tail[hb]];        // middle
+            r.c = ids[d->bond_tail[tb]];        // tail
+            r.kind = kTermAngle;
             r.kappa = d->angle_kappa[a];
-            r.cos0 = std::cos(d->angle_theta0[a]);
-            r.sin0 = std::sin(d->angle_theta0[a]);
+            r.p0 = std::cos(d->angle_theta0[a]);
+            r.p1 = std::sin(d->angle_theta0[a]);
             w.angles.push_back(r);
         }
     }
     for (const auto &b : w.bonds)
-        if (b.head >= w.n || b.tail >= w.n) return fail(PDPD_ERR_INVALID, "bond bead id out of range");
+        if (b.a >= w.n || b.b >= w.n) return fail(PDPD_ERR_INVALID, "bond bead id out of range");
     for (const auto &a : w.angles)
-        if (a.head >= w.n || a.mid >= w.n || a.tail >= w.n) return fail(PDPD_ERR_INVALID, "angle bead id out of range");
+        if (a.a >= w.n || a.b >= w.n || a.c >= w.n) return fail(PDPD_ERR_INVALID, "angle bead id out of range");
 
     int slot = ctx->next_slot++;
     w.slot = slot;
@@ -872,6 +814,14 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         ctx->graph_version = ~0ull;
         return PDPD_OK;
     }
+    if (k == "force_variant") {
+        uint32_t n;
+        if (parse_u32(v, 0, kForceVariants - 1, &n))
+            return fail(PDPD_ERR_INVALID, "force_variant must be an integer in [0, " + std::to_string(kForceVariants - 1) + "]");
+        ctx->force_variant = n;
+        ctx->graph_version = ~0ull;
+        return PDPD_OK;
+    }
     return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
 }
 
@@ -881,6 +831,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     std::string k(key), v;
     if (k == "graph_steps") v = std::to_string(ctx->graph_steps);
     else if (k == "force_grid") v = std::to_string(ctx->force_grid);
+    else if (k == "force_variant") v = std::to_string(ctx->force_variant);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -893,6 +844,8 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "graph_steps=" + std::to_string(ctx->graph_steps) + "\tsteps per captured hipGraph (even; 0 = direct launches)\n";
     s += "force_grid=" + std::to_string(ctx->force_grid) +
          "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
+    s += "force_variant=" + std::to_string(ctx->force_variant) +
+         "\tpair-force kernel variant: 0 = 2 loads per row in flight at 5 waves/SIMD, 1 = 3 at 4, 2 = 2 at 4\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }
@@ -936,7 +889,7 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps)
     ctx->stats.k_scatter_ms = acc[2] / nsteps;
     ctx->stats.k_reorder_ms = acc[3] / nsteps;
     ctx->stats.k_force_ms = acc[4] / nsteps;
-    ctx->stats.k_bonds_ms = acc[5] / nsteps;
+    ctx->stats.k_stationary_ms = acc[5] / nsteps;
     ctx->stats.profiled_steps = nsteps;
     return check_flags(ctx);
 }
@@ -958,13 +911,15 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     HIP_TRY(dev_alloc(sv, nb));
     HIP_TRY(dev_alloc(sf, nb));
     const int c0 = ctx->cur;
-    HIP_TRY(hipMemcpy(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(sv.p, ctx->vel[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(sf.p, ctx->frc[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
+    // save and restore on the engine's stream: the stream is non-blocking, so a
+    // plain hipMemcpy would not be ordered with the trial steps
+    HIP_TRY(hipMemcpyAsync(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(sv.p, ctx->vel[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(sf.p, ctx->frc[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     auto restore = [&]() -> int {
-        HIP_TRY(hipMemcpy(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpyAsync(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
         ctx->cur = c0;
         return PDPD_OK;
     };
@@ -976,43 +931,52 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     }
     const uint32_t grids[] = {0u, (uint32_t)cus * 4, (uint32_t)cus * 8, (uint32_t)cus * 16};
     const uint32_t graphs[] = {10u, 20u, 50u};
-    const uint32_t saved_grid = ctx->force_grid, saved_graph = ctx->graph_steps;
-    uint32_t best_grid = saved_grid, best_graph = saved_graph;
+    uint32_t best_grid = ctx->force_grid, best_graph = ctx->graph_steps, best_var = ctx->force_variant;
     double best_ms = 1e30;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    for (uint32_t g : grids) {
-        for (uint32_t gs : graphs) {
-            ctx->force_grid = g;
-            ctx->graph_steps = gs;
-            ctx->graph_version = ~0ull;
-            const uint32_t steps = std::max(steps_per_trial, gs);
-            for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
-                if ((rc = restore())) return rc;
-                if ((rc = load_counters(ctx, steps, nullptr))) return rc;
-                if (pass) HIP_TRY(hipEventRecord(e0, ctx->stream));
-                if ((rc = enqueue_steps(ctx, steps))) return rc;
-                if (pass) HIP_TRY(hipEventRecord(e1, ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
-            }
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            const double per = ms / steps;
-            if (per < best_ms) { best_ms = per; best_grid = g; best_graph = gs; }
+    auto trial = [&](uint32_t g, uint32_t var, uint32_t gs) -> int {
+        ctx->force_grid = g;
+        ctx->force_variant = var;
+        ctx->graph_steps = gs;
+        ctx->graph_version = ~0ull;
+        const uint32_t steps = std::max(steps_per_trial, gs);
+        for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
+            int r;
+            if ((r = restore())) return r;
+            if ((r = load_counters(ctx, steps, nullptr))) return r;
+            if (pass) HIP_TRY(hipEventRecord(e0, ctx->stream));
+            if ((r = enqueue_steps(ctx, steps))) return r;
+            if (pass) HIP_TRY(hipEventRecord(e1, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
         }
-    }
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / steps;
+        if (per < best_ms) { best_ms = per; best_grid = g; best_var = var; best_graph = gs; }
+        return PDPD_OK;
+    };
+    // pair-force grid x kernel variant at 20 steps per graph, then the graph length
+    for (uint32_t g : grids)
+        for (uint32_t var = 0; var < kForceVariants; var++)
+            if ((rc = trial(g, var, 20u))) return rc;
+    const uint32_t g_best = best_grid, v_best = best_var;
+    for (uint32_t gs : graphs)
+        if (gs != 20u && (rc = trial(g_best, v_best, gs))) return rc;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if ((rc = restore())) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     sp.release(); sv.release(); sf.release();
     ctx->force_grid = best_grid;
+    ctx->force_variant = best_var;
     ctx->graph_steps = best_graph;
     ctx->graph_version = ~0ull;
-    (void)saved_grid; (void)saved_graph;
     if ((rc = check_flags(ctx))) return rc;
     if (choice && choice_len)
-        snprintf(choice, choice_len, "force_grid=%u graph_steps=%u ms_per_step=%.5f", best_grid, best_graph, best_ms);
+        snprintf(choice, choice_len, "force_grid=%u force_variant=%u graph_steps=%u ms_per_step=%.5f", best_grid,
+                 best_var, best_graph, best_ms);
     return PDPD_OK;
 }
 

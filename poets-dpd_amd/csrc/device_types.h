@@ -10,10 +10,9 @@ namespace pdpd {
 constexpr int kMaxTypes = 16;          // 4-bit type field of the bead hash
 constexpr int kBeadThreads = 256;      // bead-parallel kernels
 constexpr int kForceBeads = 64;        // beads per pair-force workgroup (one per lane of a wave)
-constexpr int kForcePlanes = 3;        // waves per pair-force workgroup: one per x-plane of the stencil
-constexpr int kForceThreads = kForceBeads * kForcePlanes;
-constexpr int kPairList = 16;          // accepted candidates buffered per lane before evaluation
-constexpr int kScanThreads = 1024;
+constexpr int kForcePlanes = 3;        // pair waves per pair-force workgroup: one per x-plane of the stencil
+constexpr int kForceThreads = kForceBeads * (kForcePlanes + 1);   // + one wave for the bonded terms
+constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 8;
 constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan tile
 constexpr uint32_t kMaxLayoutBeads = 1u << 26;              // pair-list entries carry 26-bit slots
@@ -49,20 +48,16 @@ struct BeadBlock {
     uint32_t pad;
 };
 
-// Bonded topology. Bond and angle records hold global bead indices, in the
-// reference's order (per polymer: the type's bonds, then its bond pairs).
-// Bonded terms are evaluated in FP64 (see k_bonds).
-struct BondRec { uint32_t head, tail; double kappa, r0; };
-struct AngleRec { uint32_t head, mid, tail, pad; double kappa, cos0, sin0; };
+// Bonded terms: one record per Hookean bond or angle, global bead indices.
+//   kind 0 (Hookean): a = head, b = tail, p0 = r0
+//   kind 1 (angle):   a = head, b = middle, c = tail, p0 = cos(theta0), p1 = sin(theta0)
+// Bonded terms are evaluated in FP64 (term_force).
+struct TermRec { uint32_t a, b, c, kind; double kappa, p0, p1; };
+constexpr uint32_t kTermBond = 0, kTermAngle = 1;
 
-// A bead that takes part in bonds: its terms are bead_terms[t0, t1), each
-// (role << 29) | record index, in the order the per-polymer loop applies them.
-struct BondedBead { uint32_t bead; uint32_t t0, t1; int32_t world; };
-
-// A group of whole bonded beads with at most kBondThreads terms: one lane per
-// term, then one lane per bead sums its terms in order.
-constexpr int kBondThreads = 256;
-struct BondBlock { uint32_t bead_begin, bead_end, term_begin, term_end; int32_t world; uint32_t pad[3]; };
+// Each bead's terms are bead_terms[term_start[bead], term_start[bead + 1]),
+// (role << 29) | record index, in the order the reference's per-polymer loop
+// applies them (the polymer's bonds, then its bond pairs).
 constexpr uint32_t kRoleShift = 29;
 constexpr uint32_t kRoleBondHead = 0, kRoleBondTail = 1, kRoleAngleHead = 2, kRoleAngleTail = 3,
                    kRoleAngleMid = 4;
@@ -89,11 +84,9 @@ struct DeviceState {
     WorldCtr *ctr;
     BeadBlock *bead_blocks;    // kBeadThreads beads per block
     BeadBlock *force_blocks;   // kForceBeads beads per block
-    BondedBead *bonded;
-    BondBlock *bond_blocks;
+    uint32_t *term_start;      // n_beads + 1, by global bead index
     uint32_t *bead_terms;
-    BondRec *bonds;
-    AngleRec *angles;
+    TermRec *terms;
     StationaryBead *stationary_beads;
 };
 

@@ -14,13 +14,15 @@ struct StepLaunch {
     uint32_t n_worlds;
     uint32_t n_bead_blocks;
     uint32_t n_force_blocks;
-    uint32_t n_bond_blocks;
     uint32_t n_stationary;   // beads of stationary types
     uint32_t force_grid;     // 0 = one workgroup per force block, else persistent grid size
+    uint32_t force_variant;  // pair-force kernel variant (loads in flight / occupancy), same results
 };
 
+constexpr uint32_t kForceVariants = 3;
+
 // Events recorded between the kernels of one step (profiling runs only):
-// [0] integrate [1] scan [2] scatter [3] reorder [4] force [5] bonds [6]
+// [0] integrate [1] scan [2] scatter [3] reorder [4] force (+ bonded terms) [5] stationary [6]
 struct KernelEvents {
     hipEvent_t ev[7];
 };

@@ -3,21 +3,20 @@
 // Step (one set of launches advances every resident world):
 //   k_integrate   finish-kick + drift + half-kick + wrap, cell key, arrival rank
 //                 (dpd_maths_core_half_step_raw.hpp:42-72,114-124 fused as
-//                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does)
+//                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does); advances
+//                 the per-world step counters
 //   k_scan        single-pass exclusive scan of the cell counts (look-back)
 //   k_scatter     sorted position of every bead's slot
 //   k_reorder     stable gather into the other buffer, bead-index order in a cell
-//   k_force       pair forces: per workgroup 64 consecutive beads x 3 waves, one
-//                 wave per x-plane of the 27-cell stencil; neighbours are read
-//                 straight from the cell-sorted arrays (contiguous z-rows); each
-//                 lane buffers the candidates inside the cutoff and then
-//                 evaluates them; register accumulation, no atomics
-//                 (dpd_maths_core_half_step_raw.hpp:129-216; RNG of
-//                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79)
-//   k_bonds       Hookean + angle terms per bonded bead, in the per-polymer
-//                 order of naive_dpd_engine_half_merge.hpp:376-385 (terms of
-//                 :638-690 / dpd_maths_core_half_step.hpp:178-231), then the
-//                 per-world step counter
+//   k_force       per workgroup 64 consecutive beads: three waves for the pair
+//                 forces, one per x-plane of the 27-cell stencil, neighbours read
+//                 straight from the cell-sorted arrays; register accumulation, no
+//                 atomics (dpd_maths_core_half_step_raw.hpp:129-216; RNG of
+//                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79); a fourth wave
+//                 evaluates the home beads' Hookean and angle terms in FP64 in
+//                 the per-polymer order of naive_dpd_engine_half_merge.hpp:376-385
+//                 (terms of :638-690 / dpd_maths_core_half_step.hpp:178-231)
+//   k_stationary  (only with stationary bead types) v = 0 after the pair forces
 // The final half-kick v += f dt/2 of a step is folded into the next k_integrate
 // (the engine stores w = v - f dt/2).
 //
@@ -130,11 +129,23 @@ __device__ __forceinline__ int cell_coord(float x, int L)
 
 // ---------------------------------------------------------- integrate ----
 
-__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur, uint32_t n_tiles, uint32_t n_blocks)
+__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur, uint32_t n_tiles, uint32_t n_blocks,
+                                                           WorldCtr *__restrict__ ctr_next, uint32_t n_worlds)
 {
     if (blockIdx.x == 0) {       // reset the scan's look-back state for this step
         for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x) S.scan_tiles[i] = 0ull;
         if (threadIdx.x == 0) *S.scan_ticket = 0u;
+    }
+    {
+        // step counters: this step reads S.ctr; the next one reads ctr_next,
+        // which nothing reads during this step
+        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n_worlds) {
+            const WorldCtr c = S.ctr[i];
+            WorldCtr o = c;
+            if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
+            ctr_next[i] = o;
+        }
     }
     if (blockIdx.x >= n_blocks) return;
     const BeadBlock blk = S.bead_blocks[blockIdx.x];
@@ -179,7 +190,23 @@ __global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int c
     if (bad) atomicOr(S.flags, PDPD_FLAG_BAD_POSITION);
     const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
     const uint32_t key = W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2);
-    const uint32_t r = atomicAdd(&S.cell_count[key], 1u);
+    // arrival rank in the cell. The lanes hold beads in the last step's cell
+    // order, so equal keys come in runs of consecutive lanes: one atomic per
+    // run reserves the run's ranks (active lanes are a prefix of the wave).
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    // both ballots in converged code: a ballot inside a branch sees only the
+    // lanes that take the branch
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long lanes = __ballot(1);
+    const unsigned long long upto = (2ull << lane) - 1ull;          // lanes 0..lane
+    const int start = 63 - __clzll((long long)(heads & upto));       // first lane of this run
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll((long long)above) - 1 : (int)__popcll(lanes);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&S.cell_count[key], (uint32_t)(next - lane));
+    const uint32_t r = __shfl(base, start, 64) + (uint32_t)(lane - start);
     S.key[s] = key;
     S.rank[s] = r | (active ? 0u : kInactive);
 }
@@ -212,33 +239,49 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ co
     }
     uint32_t agg;
     uint32_t ex = block_scan<kScanThreads / 64>(acc, s_wsum, &agg);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        // look-back by the first wave: lane l reads tile (base - l); the window
+        // moves back 64 tiles at a time until it holds an inclusive prefix (tile
+        // -1 counts as inclusive 0)
+        const int lane = threadIdx.x;
         uint32_t prefix = 0;
         if (tile == 0) {
-            __hip_atomic_store(&tiles[0], kTileIncl | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(&tiles[0], kTileIncl | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&tiles[tile], kTileAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int j = (int)tile - 1;
+            if (lane == 0) __hip_atomic_store(&tiles[tile], kTileAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int base = (int)tile - 1;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const unsigned long long st = __hip_atomic_load(&tiles[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+                const int j = base - lane;
+                const unsigned long long st =
+                    j >= 0 ? __hip_atomic_load(&tiles[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kTileIncl;
                 const unsigned long long status = st & 0xffffffff00000000ull;
-                if (status == 0ull) {
+                const unsigned long long incl = __ballot(status == kTileIncl);
+                const unsigned long long none = __ballot(status == 0ull);
+                const int first = incl ? __ffsll((long long)incl) - 1 : 64;
+                const unsigned long long upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+                if (none & upto) {                          // a tile in the window has not published yet
                     if (++spins > kScanSpinLimit) {
-                        atomicOr(flags, PDPD_FLAG_SCAN_TIMEOUT);
+                        if (lane == 0) atomicOr(flags, PDPD_FLAG_SCAN_TIMEOUT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += (uint32_t)st;
-                if (status == kTileIncl) break;
-                j--;
+                uint32_t val = lane <= first ? (uint32_t)st : 0u;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+                prefix += val;
+                if (first < 64) break;
+                base -= 64;
             }
-            __hip_atomic_store(&tiles[tile], kTileIncl | (prefix + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&tiles[tile], kTileIncl | (prefix + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = prefix;
-        if (tile == n_tiles - 1) cell_start[n] = prefix + agg;
+        if (lane == 0) {
+            s_prefix = prefix;
+            if (tile == n_tiles - 1) cell_start[n] = prefix + agg;
+        }
     }
     __syncthreads();
     ex += s_prefix;
@@ -286,147 +329,6 @@ __global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur
     S.slot_of_bead[me] = dst;
 }
 
-// --------------------------------------------------------- pair force ----
-
-// Shift codes of a candidate's periodic image, 2 bits per axis: 0 none,
-// 1 = +L, 2 = -L. Stored with the candidate's slot in the per-lane list.
-__device__ __forceinline__ float shift_of(uint32_t code, float L)
-{
-    return code == 1u ? L : (code == 2u ? -L : 0.0f);
-}
-
-__global__ __launch_bounds__(kForceThreads) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
-{
-    __shared__ uint32_t s_list[kPairList * kForceThreads];
-    __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
-
-    const int tid = threadIdx.x;
-    const int plane = tid / kForceBeads;            // wave-uniform
-    const int lane = tid % kForceBeads;
-    const float4 *__restrict__ pos = S.pos[nxt];
-    const float4 *__restrict__ vel = S.vel[nxt];
-    const uint32_t *__restrict__ cs = S.cell_start;
-
-    for (uint32_t bi = blockIdx.x; bi < n_blocks; bi += gridDim.x) {
-        const BeadBlock blk = S.force_blocks[bi];
-        if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world
-        const WorldDev *__restrict__ W = &S.worlds[blk.world];
-        const uint32_t s = blk.begin + lane;
-        const bool live = s < blk.end;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        // a stationary home bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390);
-        // its pairs with moving beads are evaluated from the moving side
-        if (live && !((W->stationary >> (fbits(pos[s].w) >> 28)) & 1u)) {
-            const WorldCtr wc = S.ctr[blk.world];
-            const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
-            const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
-            const float scale = W->rng_scale;
-            const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
-            const float4 hp = pos[s];
-            const float4 hv = vel[s];
-            const uint32_t hh = fbits(hp.w);
-            const uint32_t trow = (hh >> 28) * kMaxTypes;
-            const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
-            uint32_t cnt = 0;
-            // one accepted pair: dpd_maths_core_half_step_raw.hpp:129-216 with the
-            // terms grouped as a wr + sgw (scale u - sgw rdotv)
-            auto pair = [&](uint32_t e, const float4 &op, const float4 &ov) {
-                const float ox = op.x + shift_of((e >> 26) & 3u, W->Lf[0]);
-                const float oy = op.y + shift_of((e >> 28) & 3u, W->Lf[1]);
-                const float oz = op.z + shift_of((e >> 30) & 3u, W->Lf[2]);
-                const float dx = hp.x - ox, dy = hp.y - oy, dz = hp.z - oz;
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float rr = sqrtf(r2);
-                const float inv_r = 1.0f / rr;
-                const float wr = 1.0f - rr;
-                const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
-                const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
-                const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                const uint32_t oh = fbits(op.w);
-                const uint32_t ti = trow + (oh >> 28);
-                const float sgw = W->sg[ti] * wr;
-                const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
-                const float F = fmaf(W->a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
-                fx = fmaf(ex, F, fx);
-                fy = fmaf(ey, F, fy);
-                fz = fmaf(ez, F, fz);
-            };
-            // evaluate the buffered pairs in order, loads of two pairs in flight
-            auto flush = [&]() {
-                uint32_t q = 0;
-                for (; q + 2 <= cnt; q += 2) {
-                    const uint32_t e0 = s_list[q * kForceThreads + tid];
-                    const uint32_t e1 = s_list[(q + 1) * kForceThreads + tid];
-                    const float4 op0 = pos[e0 & kSlotMask], ov0 = vel[e0 & kSlotMask];
-                    const float4 op1 = pos[e1 & kSlotMask], ov1 = vel[e1 & kSlotMask];
-                    pair(e0, op0, ov0);
-                    pair(e1, op1, ov1);
-                }
-                if (q < cnt) {
-                    const uint32_t e = s_list[q * kForceThreads + tid];
-                    pair(e, pos[e & kSlotMask], vel[e & kSlotMask]);
-                }
-                cnt = 0;
-            };
-            auto test = [&](const float4 &op, uint32_t m, bool valid, uint32_t code, float sx, float sy, float sz) {
-                const float dx = hp.x - (op.x + sx), dy = hp.y - (op.y + sy), dz = hp.z - (op.z + sz);
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (valid && r2 < 1.0f && r2 >= kMinR2 && m != s) s_list[(cnt++) * kForceThreads + tid] = m | code;
-            };
-            // candidates of cells [m0, m1) with one periodic image, four loads in
-            // flight; the list is flushed between groups so no loaded candidate is
-            // live across a flush
-            auto scan = [&](uint32_t m0, uint32_t m1, uint32_t code, float sx, float sy, float sz) {
-                for (uint32_t m = m0; m < m1; m += 4) {
-                    if (cnt > kPairList - 4) flush();
-                    const uint32_t last = m1 - 1;
-                    const float4 p0 = pos[m];
-                    const float4 p1 = pos[min(m + 1, last)];
-                    const float4 p2 = pos[min(m + 2, last)];
-                    const float4 p3 = pos[min(m + 3, last)];
-                    test(p0, m, true, code, sx, sy, sz);
-                    test(p1, m + 1, m + 1 < m1, code, sx, sy, sz);
-                    test(p2, m + 2, m + 2 < m1, code, sx, sy, sz);
-                    test(p3, m + 3, m + 3 < m1, code, sx, sy, sz);
-                }
-            };
-            int ox = cx + plane - 1;
-            uint32_t codex = 0;
-            if (ox < 0) { ox += L0; codex = 2u; } else if (ox >= L0) { ox -= L0; codex = 1u; }
-            const float sx = shift_of(codex, W->Lf[0]);
-#pragma unroll 1
-            for (int dyi = -1; dyi <= 1; dyi++) {
-                int oy = cy + dyi;
-                uint32_t codey = 0;
-                if (oy < 0) { oy += L1; codey = 2u; } else if (oy >= L1) { oy -= L1; codey = 1u; }
-                const float sy = shift_of(codey, W->Lf[1]);
-                const uint32_t row = W->cell_off + (uint32_t)((ox * L1 + oy) * L2);
-                const uint32_t cxy = (codex << 26) | (codey << 28);
-                // z order -1, 0, +1: an image below 0 first, the contiguous run, an image above L-1 last
-                if (cz == 0) scan(cs[row + L2 - 1], cs[row + L2], cxy | (2u << 30), sx, sy, -W->Lf[2]);
-                const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz < L2 - 1 ? cz + 1 : L2 - 1;
-                scan(cs[row + zlo], cs[row + zhi + 1], cxy, sx, sy, 0.0f);
-                if (cz == L2 - 1) scan(cs[row], cs[row + 1], cxy | (1u << 30), sx, sy, W->Lf[2]);
-            }
-            flush();
-        }
-        // combine the three planes in order: (p0 + p1) + p2
-        if (plane > 0) {
-            s_part[plane - 1][0][lane] = fx;
-            s_part[plane - 1][1][lane] = fy;
-            s_part[plane - 1][2][lane] = fz;
-        }
-        __syncthreads();
-        if (plane == 0 && live) {
-            fx = (fx + s_part[0][0][lane]) + s_part[1][0][lane];
-            fy = (fy + s_part[0][1][lane]) + s_part[1][1][lane];
-            fz = (fz + s_part[0][2][lane]) + s_part[1][2][lane];
-            S.frc[nxt][s] = make_float4(fx, fy, fz, 0.0f);
-        }
-        __syncthreads();
-    }
-}
-
 // -------------------------------------------------------------- bonds ----
 
 __device__ __forceinline__ double min_image(double d, double len)
@@ -442,20 +344,16 @@ __device__ __forceinline__ double min_image(double d, double len)
 // chains. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
 // dpd_maths_core_half_step.hpp:178-231 with the prefactor floor of
 // naive_dpd_engine.hpp:681-682 (the argument clamped at 0).
-__device__ void bond_term(const DeviceState &S, const float4 *__restrict__ pos, const WorldDev *__restrict__ W,
-                          uint32_t code, double &cx, double &cy, double &cz)
+__device__ __forceinline__ void term_force(uint32_t role, const TermRec &R, const float3 &xa, const float3 &xb,
+                                           const float3 &xc, double Lx, double Ly, double Lz, double &cx, double &cy,
+                                           double &cz)
 {
-    const uint32_t role = code >> kRoleShift;
-    const uint32_t idx = code & ((1u << kRoleShift) - 1u);
-    const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
     if (role <= kRoleBondTail) {
-        const BondRec b = S.bonds[idx];
-        const float4 xh = pos[S.slot_of_bead[b.head]], xt = pos[S.slot_of_bead[b.tail]];
-        const double dx = min_image((double)xt.x - (double)xh.x, Lx);
-        const double dy = min_image((double)xt.y - (double)xh.y, Ly);
-        const double dz = min_image((double)xt.z - (double)xh.z, Lz);
+        const double dx = min_image((double)xb.x - (double)xa.x, Lx);
+        const double dy = min_image((double)xb.y - (double)xa.y, Ly);
+        const double dz = min_image((double)xb.z - (double)xa.z, Lz);
         const double r = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
-        const double hf = b.kappa * (b.r0 - r);
+        const double hf = R.kappa * (R.p0 - r);
         const double sc = hf / r;
         const double sgn = role == kRoleBondHead ? -1.0 : 1.0;   // head -= f, tail += f
         cx = sgn * (dx * sc);
@@ -463,12 +361,10 @@ __device__ void bond_term(const DeviceState &S, const float4 *__restrict__ pos, 
         cz = sgn * (dz * sc);
         return;
     }
-    const AngleRec a = S.angles[idx];
-    const float4 xh = pos[S.slot_of_bead[a.head]], xm = pos[S.slot_of_bead[a.mid]], xt = pos[S.slot_of_bead[a.tail]];
-    const double d0x = min_image((double)xm.x - (double)xh.x, Lx), d0y = min_image((double)xm.y - (double)xh.y, Ly),
-                 d0z = min_image((double)xm.z - (double)xh.z, Lz);
-    const double d1x = min_image((double)xt.x - (double)xm.x, Lx), d1y = min_image((double)xt.y - (double)xm.y, Ly),
-                 d1z = min_image((double)xt.z - (double)xm.z, Lz);
+    const double d0x = min_image((double)xb.x - (double)xa.x, Lx), d0y = min_image((double)xb.y - (double)xa.y, Ly),
+                 d0z = min_image((double)xb.z - (double)xa.z, Lz);
+    const double d1x = min_image((double)xc.x - (double)xb.x, Lx), d1y = min_image((double)xc.y - (double)xb.y, Ly),
+                 d1z = min_image((double)xc.z - (double)xb.z, Lz);
     const double r01 = sqrt(fma(d0z, d0z, fma(d0y, d0y, d0x * d0x)));
     const double r12 = sqrt(fma(d1z, d1z, fma(d1y, d1y, d1x * d1x)));
     const double mag = r01 * r12;
@@ -478,10 +374,10 @@ __device__ void bond_term(const DeviceState &S, const float4 *__restrict__ pos, 
         const double dot = fma(d0z, d1z, fma(d0y, d1y, d0x * d1x));
         const double o1 = dot / b1, o2 = dot / b2;
         const double cos2 = o1 * o2;
-        double fm = a.kappa / mag;
-        if (fabs(dot) > 0.000001 && a.sin0 > 0.0) {
+        double fm = R.kappa / mag;
+        if (fabs(dot) > 0.000001 && R.p1 > 0.0) {
             const double pref = fmax(sqrt(fmax(1.0 / cos2 - 1.0, 0.0)), 0.000001);
-            fm = fm * (a.cos0 - a.sin0 * (1.0 / pref));
+            fm = fm * (R.p0 - R.p1 * (1.0 / pref));
         }
         hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
         tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
@@ -491,56 +387,253 @@ __device__ void bond_term(const DeviceState &S, const float4 *__restrict__ pos, 
     else { cx = -(hx + tx); cy = -(hy + ty); cz = -(hz + tz); }
 }
 
-// Bonds per group of whole bonded beads: one lane per term computes its
-// contribution (all loads of a group in flight at once), then one lane per bead
-// adds its terms to the pair force in the per-polymer loop's order (bonds, then
-// bond pairs; within a bond pair head, tail, middle) in FP64 and rounds once.
-__global__ __launch_bounds__(kBondThreads) void k_bonds(DeviceState S, int nxt, int par, uint32_t n_blocks,
-                                                       uint32_t n_worlds)
+// --------------------------------------------------------- pair force ----
+
+// Shift codes of a candidate's periodic image, 2 bits per axis: 0 none,
+// 1 = +L, 2 = -L. Stored with the candidate's slot in the per-lane list.
+__device__ __forceinline__ float shift_of(uint32_t code, float L)
 {
-    __shared__ double s_c[3][kBondThreads];
-    const uint32_t tid = threadIdx.x;
-    const WorldCtr *__restrict__ ctr_in = S.ctr + par * n_worlds;
-    if (blockIdx.x < n_blocks) {
-        const BondBlock bk = S.bond_blocks[blockIdx.x];
-        if (ctr_in[bk.world].remaining > 0) {                  // uniform per block
-            const WorldDev *__restrict__ W = &S.worlds[bk.world];
-            const float4 *__restrict__ pos = S.pos[nxt];
-            const uint32_t t = bk.term_begin + tid;
-            if (t < bk.term_end) {
-                double cx, cy, cz;
-                bond_term(S, pos, W, S.bead_terms[t], cx, cy, cz);
-                s_c[0][tid] = cx;
-                s_c[1][tid] = cy;
-                s_c[2][tid] = cz;
-            }
-            __syncthreads();
-            const uint32_t b = bk.bead_begin + tid;
-            if (b < bk.bead_end) {
-                const BondedBead bb = S.bonded[b];
-                const uint32_t me = S.slot_of_bead[bb.bead];
-                float4 f = S.frc[nxt][me];
-                double ax = f.x, ay = f.y, az = f.z;
-                for (uint32_t k = bb.t0 - bk.term_begin; k < bb.t1 - bk.term_begin; k++) {
-                    ax += s_c[0][k];
-                    ay += s_c[1][k];
-                    az += s_c[2][k];
-                }
-                f.x = (float)ax;
-                f.y = (float)ay;
-                f.z = (float)az;
-                if ((S.worlds[bk.world].stationary >> (fbits(pos[me].w) >> 28)) & 1u) f = make_float4(0, 0, 0, 0);
-                S.frc[nxt][me] = f;
-            }
+    return code == 1u ? L : (code == 2u ? -L : 0.0f);
+}
+
+constexpr int kRowList = 8;       // accepted candidates buffered per row and lane
+constexpr uint32_t kXcds = 8;     // MI355X: 8 accelerator complex dies, each with its own L2
+
+// Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD i mod 8).
+// Map logical index i to a work item so that each XCD takes one contiguous
+// share of the (cell-ordered) work list and its L2 holds one region of the
+// world instead of all of it. A bijection on [0, n); with a grid-stride loop
+// whose stride is a multiple of 8, a workgroup stays on its XCD's share.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t n)
+{
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    const uint32_t x = i % kXcds, k = i / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+__device__ __forceinline__ float3 load_xyz(const float4 *__restrict__ p, uint32_t i)
+{
+    const float *q = reinterpret_cast<const float *>(p + i);
+    return make_float3(q[0], q[1], q[2]);
+}
+
+// Pair forces. A workgroup takes 64 consecutive slots of one world (one home
+// bead per lane) and has one wave per x-plane of the 27-cell stencil. A wave
+// walks the three rows (dy = -1, 0, +1) of its plane side by side: the
+// candidates of every row are loaded together, and each row keeps its own list
+// of accepted candidates and its own partial force. Per row the candidates come
+// in z order -1, 0, +1 (an image cell below 0, the contiguous run of cells, an
+// image cell above L-1) and, within a cell, in slot order. The partial sums are
+// combined as ((r0 + r1) + r2) per plane and ((p0 + p1) + p2) over planes, so
+// every force is summed in one fixed order, independent of batch and launch.
+// kScanWidth (candidate loads in flight per row and lane) and the occupancy
+// target trade registers for memory-level parallelism; every variant gives
+// bit-identical forces.
+template <int kScanWidth, int kWaves>
+__global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(
+    DeviceState S, int nxt, uint32_t n_blocks)
+{
+    __shared__ uint32_t s_list[3][kRowList][kForceBeads * kForcePlanes];
+    __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
+    __shared__ double s_bond[3][kForceBeads];
+    __shared__ uint32_t s_nterms[kForceBeads];
+    __shared__ float s_a[kMaxTypes * kMaxTypes], s_sg[kMaxTypes * kMaxTypes];
+
+    const int tid = threadIdx.x;
+    const int plane = tid / kForceBeads;            // wave-uniform
+    const int lane = tid % kForceBeads;
+    const float4 *__restrict__ pos = S.pos[nxt];
+    const float4 *__restrict__ vel = S.vel[nxt];
+    const uint32_t *__restrict__ cs = S.cell_start;
+
+    for (uint32_t li = blockIdx.x; li < n_blocks; li += gridDim.x) {
+        const BeadBlock blk = S.force_blocks[xcd_chunk(li, n_blocks)];
+        if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world
+        const WorldDev *__restrict__ W = &S.worlds[blk.world];
+        for (int i = tid; i < kMaxTypes * kMaxTypes; i += kForceThreads) {
+            s_a[i] = W->a[i];
+            s_sg[i] = W->sg[i];
         }
-    }
-    // step counters: read ctr[par], write ctr[par^1] (nothing reads that copy this step)
-    const uint32_t i = blockIdx.x * blockDim.x + tid;
-    if (i < n_worlds) {
-        const WorldCtr c = S.ctr[par * n_worlds + i];
-        WorldCtr o = c;
-        if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
-        S.ctr[(par ^ 1) * n_worlds + i] = o;
+        __syncthreads();
+        const uint32_t s = blk.begin + lane;
+        const bool live = s < blk.end;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        // a stationary home bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390);
+        // its pairs with moving beads are evaluated from the moving side
+        const bool moving = live && !((W->stationary >> (fbits(pos[s].w) >> 28)) & 1u);
+        if (plane == kForcePlanes) {
+            // the bond wave: the home bead's bonded terms, summed in FP64 in order
+            double bx = 0.0, by = 0.0, bz = 0.0;
+            uint32_t nt = 0;
+            if (moving) {
+                const uint32_t b = fbits(vel[s].w);
+                const uint32_t t0 = S.term_start[b], t1 = S.term_start[b + 1];
+                nt = t1 - t0;
+                const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+                for (uint32_t t = t0; t < t1; t += 2) {        // two terms' loads in flight
+                    const bool two = t + 1 < t1;
+                    const uint32_t e0 = S.bead_terms[t], e1 = S.bead_terms[two ? t + 1 : t];
+                    const TermRec R0 = S.terms[e0 & ((1u << kRoleShift) - 1u)];
+                    const TermRec R1 = S.terms[e1 & ((1u << kRoleShift) - 1u)];
+                    const float3 a0 = load_xyz(pos, S.slot_of_bead[R0.a]), b0 = load_xyz(pos, S.slot_of_bead[R0.b]),
+                                 c0 = load_xyz(pos, S.slot_of_bead[R0.c]);
+                    const float3 a1 = load_xyz(pos, S.slot_of_bead[R1.a]), b1 = load_xyz(pos, S.slot_of_bead[R1.b]),
+                                 c1 = load_xyz(pos, S.slot_of_bead[R1.c]);
+                    double cx, cy, cz;
+                    term_force(e0 >> kRoleShift, R0, a0, b0, c0, Lx, Ly, Lz, cx, cy, cz);
+                    bx += cx; by += cy; bz += cz;
+                    if (two) {
+                        term_force(e1 >> kRoleShift, R1, a1, b1, c1, Lx, Ly, Lz, cx, cy, cz);
+                        bx += cx; by += cy; bz += cz;
+                    }
+                }
+            }
+            s_bond[0][lane] = bx;
+            s_bond[1][lane] = by;
+            s_bond[2][lane] = bz;
+            s_nterms[lane] = nt;
+        } else if (moving) {
+            const WorldCtr wc = S.ctr[blk.world];
+            const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
+            const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
+            const float scale = W->rng_scale;
+            const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
+            const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+            const float4 hp = pos[s];
+            const float4 hv = vel[s];
+            const uint32_t hh = fbits(hp.w);
+            const uint32_t trow = (hh >> 28) * kMaxTypes;
+            const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
+
+            // the plane's three rows; per row a stream of at most two slot ranges,
+            // [ra, ra + na) then [rb, rb + nb)
+            int ox = cx + plane - 1;
+            uint32_t codex = 0;
+            if (ox < 0) { ox += L0; codex = 2u; } else if (ox >= L0) { ox -= L0; codex = 1u; }
+            const float sx = shift_of(codex, Lx);
+            const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz < L2 - 1 ? cz + 1 : L2 - 1;
+            const bool pre = cz == 0, post = cz == L2 - 1;
+            const int zi = pre ? L2 - 1 : 0;                 // the image cell, if any
+            const float szA = pre ? -Lz : 0.0f, szB = post ? Lz : 0.0f;
+            const uint32_t zcA = pre ? (2u << 30) : 0u, zcB = post ? (1u << 30) : 0u;
+            uint32_t ra[3], na[3], rb[3], nb[3], cxy[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                int oy = cy + r - 1;
+                uint32_t codey = 0;
+                if (oy < 0) { oy += L1; codey = 2u; } else if (oy >= L1) { oy -= L1; codey = 1u; }
+                cxy[r] = (codex << 26) | (codey << 28);
+                const uint32_t row = W->cell_off + (uint32_t)((ox * L1 + oy) * L2);
+                const uint32_t mb = cs[row + zlo], me = cs[row + zhi + 1];
+                const uint32_t ib = (pre || post) ? cs[row + zi] : 0u;
+                const uint32_t ie = (pre || post) ? cs[row + zi + 1] : 0u;
+                ra[r] = pre ? ib : mb;
+                na[r] = pre ? ie - ib : me - mb;
+                rb[r] = pre ? mb : ib;
+                nb[r] = pre ? me - mb : ie - ib;
+            }
+
+            float px[3] = {0.0f, 0.0f, 0.0f}, py[3] = {0.0f, 0.0f, 0.0f}, pz[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t cnt[3] = {0u, 0u, 0u};
+            // one accepted pair: dpd_maths_core_half_step_raw.hpp:129-216 with the
+            // terms grouped as a wr + sgw (scale u - sgw rdotv)
+            auto pair = [&](uint32_t e, const float4 &op, const float4 &ov, float &ax, float &ay, float &az) {
+                const float ox_ = op.x + shift_of((e >> 26) & 3u, Lx);
+                const float oy_ = op.y + shift_of((e >> 28) & 3u, Ly);
+                const float oz_ = op.z + shift_of((e >> 30) & 3u, Lz);
+                const float dx = hp.x - ox_, dy = hp.y - oy_, dz = hp.z - oz_;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float rr = sqrtf(r2);
+                const float inv_r = 1.0f / rr;
+                const float wr = 1.0f - rr;
+                const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+                const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
+                const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+                const uint32_t oh = fbits(op.w);
+                const uint32_t ti = trow + (oh >> 28);
+                const float sgw = s_sg[ti] * wr;
+                const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
+                const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                ax = fmaf(ex, F, ax);
+                ay = fmaf(ey, F, ay);
+                az = fmaf(ez, F, az);
+            };
+            // evaluate row r's buffered pairs in order, two in flight
+            auto flush = [&](int r) {
+                uint32_t q = 0;
+                for (; q + 2 <= cnt[r]; q += 2) {
+                    const uint32_t e0 = s_list[r][q][tid], e1 = s_list[r][q + 1][tid];
+                    const float4 op0 = pos[e0 & kSlotMask], ov0 = vel[e0 & kSlotMask];
+                    const float4 op1 = pos[e1 & kSlotMask], ov1 = vel[e1 & kSlotMask];
+                    pair(e0, op0, ov0, px[r], py[r], pz[r]);
+                    pair(e1, op1, ov1, px[r], py[r], pz[r]);
+                }
+                if (q < cnt[r]) {
+                    const uint32_t e = s_list[r][q][tid];
+                    pair(e, pos[e & kSlotMask], vel[e & kSlotMask], px[r], py[r], pz[r]);
+                }
+                cnt[r] = 0;
+            };
+            const uint32_t len0 = na[0] + nb[0], len1 = na[1] + nb[1], len2 = na[2] + nb[2];
+            const uint32_t maxlen = max(len0, max(len1, len2));
+            for (uint32_t k = 0; k < maxlen; k += kScanWidth) {
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    if (cnt[r] > kRowList - kScanWidth) flush(r);
+                uint32_t slot[3][kScanWidth];
+                float3 p[3][kScanWidth];
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const uint32_t len = na[r] + nb[r];
+#pragma unroll
+                    for (int j = 0; j < kScanWidth; j++) {
+                        const uint32_t q = k + j;
+                        // beyond the row's stream: the home slot, a valid address that is never accepted
+                        slot[r][j] = q >= len ? s : (q < na[r] ? ra[r] + q : rb[r] + (q - na[r]));
+                        p[r][j] = load_xyz(pos, slot[r][j]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+#pragma unroll
+                    for (int j = 0; j < kScanWidth; j++) {
+                        const uint32_t q = k + j;
+                        const bool inA = q < na[r];
+                        const float ozs = inA ? szA : szB;
+                        const float sy = shift_of((cxy[r] >> 28) & 3u, Ly);
+                        const float dx = hp.x - (p[r][j].x + sx), dy = hp.y - (p[r][j].y + sy),
+                                    dz = hp.z - (p[r][j].z + ozs);
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        if (r2 < 1.0f && r2 >= kMinR2 && slot[r][j] != s)
+                            s_list[r][cnt[r]++][tid] = slot[r][j] | cxy[r] | (inA ? zcA : zcB);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++) flush(r);
+            fx = (px[0] + px[1]) + px[2];
+            fy = (py[0] + py[1]) + py[2];
+            fz = (pz[0] + pz[1]) + pz[2];
+        }
+        // combine the three planes in order: (p0 + p1) + p2
+        if (plane > 0 && plane < kForcePlanes) {
+            s_part[plane - 1][0][lane] = fx;
+            s_part[plane - 1][1][lane] = fy;
+            s_part[plane - 1][2][lane] = fz;
+        }
+        __syncthreads();
+        if (plane == 0 && live) {
+            fx = (fx + s_part[0][0][lane]) + s_part[1][0][lane];
+            fy = (fy + s_part[0][1][lane]) + s_part[1][1][lane];
+            fz = (fz + s_part[0][2][lane]) + s_part[1][2][lane];
+            if (s_nterms[lane]) {      // pair force + bonded terms, rounded once
+                fx = (float)((double)fx + s_bond[0][lane]);
+                fy = (float)((double)fy + s_bond[1][lane]);
+                fz = (float)((double)fz + s_bond[2][lane]);
+            }
+            S.frc[nxt][s] = make_float4(fx, fy, fz, 0.0f);
+        }
+        __syncthreads();
     }
 }
 
@@ -593,7 +686,9 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     const uint32_t nt = scan_tiles(L.n_cells);
 
     rec(0);
-    k_integrate<<<std::max<uint32_t>(1, L.n_bead_blocks), kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks);
+    const uint32_t ig = std::max<uint32_t>(std::max<uint32_t>(1, L.n_bead_blocks), ceil_div(L.n_worlds, kBeadThreads));
+    k_integrate<<<ig, kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks, S.ctr + (par ^ 1) * L.n_worlds,
+                                              L.n_worlds);
     rec(1);
     k_scan<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, S.scan_tiles, S.scan_ticket, nt,
                                          S.flags);
@@ -604,12 +699,16 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
     rec(4);
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
-    if (fg) k_force<<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks);
+    if (fg) {
+        switch (L.force_variant) {
+        case 1: k_force<3, 4><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        case 2: k_force<2, 4><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        default: k_force<2, 5><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        }
+    }
     rec(5);
     if (L.n_stationary)
         k_stationary<<<ceil_div(L.n_stationary, kBeadThreads), kBeadThreads, 0, st>>>(Sp, cur ^ 1, L.n_stationary);
-    const uint32_t bg = std::max<uint32_t>(L.n_bond_blocks, ceil_div(L.n_worlds, kBondThreads));
-    k_bonds<<<bg, kBondThreads, 0, st>>>(S, cur ^ 1, par, L.n_bond_blocks, L.n_worlds);
     rec(6);
     return hipGetLastError();
 }

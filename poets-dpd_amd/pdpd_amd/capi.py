@@ -52,7 +52,7 @@ class Stats(C.Structure):
                 ("bead_steps", C.c_uint64), ("flags", C.c_uint32), ("n_worlds", C.c_uint32),
                 ("n_beads", C.c_uint64), ("n_force_blocks", C.c_uint32),
                 ("k_integrate_ms", C.c_double), ("k_scan_ms", C.c_double), ("k_scatter_ms", C.c_double),
-                ("k_reorder_ms", C.c_double), ("k_force_ms", C.c_double), ("k_bonds_ms", C.c_double),
+                ("k_reorder_ms", C.c_double), ("k_force_ms", C.c_double), ("k_stationary_ms", C.c_double),
                 ("profiled_steps", C.c_uint64)]
 
     def as_dict(self):
