@@ -476,6 +476,8 @@ int check_flags(pdpd_ctx *ctx)
     HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (f & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
+    if (f & PDPD_FLAG_DENSE_ROW)
+        return fail(PDPD_ERR_STATE, "three neighbouring cells hold more than 65535 beads; forces were not computed");
     return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
 }
 
@@ -845,7 +847,8 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "force_grid=" + std::to_string(ctx->force_grid) +
          "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
     s += "force_variant=" + std::to_string(ctx->force_variant) +
-         "\tpair-force kernel variant: 0 = 2 loads per row in flight at 5 waves/SIMD, 1 = 3 at 4, 2 = 2 at 4\n";
+         "\tpair-force kernel variant: 0 = 2 loads per row in flight, 5 waves/SIMD, bond wave; 1 = 3, 4, bond wave;"
+         " 2 = 2, 5, bonds in the middle-plane wave; 3 = 2, 6, bonds in the middle-plane wave\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }

@@ -19,7 +19,7 @@ struct StepLaunch {
     uint32_t force_variant;  // pair-force kernel variant (loads in flight / occupancy), same results
 };
 
-constexpr uint32_t kForceVariants = 3;
+constexpr uint32_t kForceVariants = 4;
 
 // Events recorded between the kernels of one step (profiling runs only):
 // [0] integrate [1] scan [2] scatter [3] reorder [4] force (+ bonded terms) [5] stationary [6]

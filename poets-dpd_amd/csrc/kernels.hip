@@ -429,11 +429,47 @@ __device__ __forceinline__ float3 load_xyz(const float4 *__restrict__ p, uint32_
 // kScanWidth (candidate loads in flight per row and lane) and the occupancy
 // target trade registers for memory-level parallelism; every variant gives
 // bit-identical forces.
-template <int kScanWidth, int kWaves>
-__global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(
-    DeviceState S, int nxt, uint32_t n_blocks)
+// The bonded terms of the bead in slot s, summed in FP64 in the per-polymer
+// order into s_bond / s_nterms (lane = the bead's lane in its workgroup).
+__device__ __forceinline__ void bonded_terms(const DeviceState &S, const WorldDev *__restrict__ W,
+                                             const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                             uint32_t s, bool moving, int lane, double (*s_bond)[kForceBeads],
+                                             uint32_t *s_nterms)
 {
-    __shared__ uint32_t s_list[3][kRowList][kForceBeads * kForcePlanes];
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    uint32_t nt = 0;
+    if (moving) {
+        const uint32_t b = fbits(vel[s].w);
+        const uint32_t t0 = S.term_start[b], t1 = S.term_start[b + 1];
+        nt = t1 - t0;
+        const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+        for (uint32_t t = t0; t < t1; t++) {
+            const uint32_t e = S.bead_terms[t];
+            const TermRec R = S.terms[e & ((1u << kRoleShift) - 1u)];
+            const float3 xa = load_xyz(pos, S.slot_of_bead[R.a]), xb = load_xyz(pos, S.slot_of_bead[R.b]),
+                         xc = load_xyz(pos, S.slot_of_bead[R.c]);
+            double cx, cy, cz;
+            term_force(e >> kRoleShift, R, xa, xb, xc, Lx, Ly, Lz, cx, cy, cz);
+            bx += cx; by += cy; bz += cz;
+        }
+    }
+    s_bond[0][lane] = bx;
+    s_bond[1][lane] = by;
+    s_bond[2][lane] = bz;
+    s_nterms[lane] = nt;
+}
+
+// kBondWave: a fourth wave evaluates the bonded terms beside the pair waves;
+// otherwise the middle plane's wave evaluates them after its pairs (three waves
+// per workgroup, more workgroups resident). Same sums either way.
+template <int kScanWidth, int kWaves, bool kBondWave>
+__global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
+__attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
+{
+    constexpr int kThreads = kBondWave ? kForceThreads : kForceBeads * kForcePlanes;
+    // accepted candidates as positions in their row's stream (the slot and the
+    // periodic-image code follow from the row), 16 bits each
+    __shared__ uint16_t s_list[3][kRowList][kForceBeads * kForcePlanes];
     __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
     __shared__ double s_bond[3][kForceBeads];
     __shared__ uint32_t s_nterms[kForceBeads];
@@ -450,7 +486,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
         const BeadBlock blk = S.force_blocks[xcd_chunk(li, n_blocks)];
         if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world
         const WorldDev *__restrict__ W = &S.worlds[blk.world];
-        for (int i = tid; i < kMaxTypes * kMaxTypes; i += kForceThreads) {
+        for (int i = tid; i < kMaxTypes * kMaxTypes; i += kThreads) {
             s_a[i] = W->a[i];
             s_sg[i] = W->sg[i];
         }
@@ -461,37 +497,8 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
         // a stationary home bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390);
         // its pairs with moving beads are evaluated from the moving side
         const bool moving = live && !((W->stationary >> (fbits(pos[s].w) >> 28)) & 1u);
-        if (plane == kForcePlanes) {
-            // the bond wave: the home bead's bonded terms, summed in FP64 in order
-            double bx = 0.0, by = 0.0, bz = 0.0;
-            uint32_t nt = 0;
-            if (moving) {
-                const uint32_t b = fbits(vel[s].w);
-                const uint32_t t0 = S.term_start[b], t1 = S.term_start[b + 1];
-                nt = t1 - t0;
-                const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
-                for (uint32_t t = t0; t < t1; t += 2) {        // two terms' loads in flight
-                    const bool two = t + 1 < t1;
-                    const uint32_t e0 = S.bead_terms[t], e1 = S.bead_terms[two ? t + 1 : t];
-                    const TermRec R0 = S.terms[e0 & ((1u << kRoleShift) - 1u)];
-                    const TermRec R1 = S.terms[e1 & ((1u << kRoleShift) - 1u)];
-                    const float3 a0 = load_xyz(pos, S.slot_of_bead[R0.a]), b0 = load_xyz(pos, S.slot_of_bead[R0.b]),
-                                 c0 = load_xyz(pos, S.slot_of_bead[R0.c]);
-                    const float3 a1 = load_xyz(pos, S.slot_of_bead[R1.a]), b1 = load_xyz(pos, S.slot_of_bead[R1.b]),
-                                 c1 = load_xyz(pos, S.slot_of_bead[R1.c]);
-                    double cx, cy, cz;
-                    term_force(e0 >> kRoleShift, R0, a0, b0, c0, Lx, Ly, Lz, cx, cy, cz);
-                    bx += cx; by += cy; bz += cz;
-                    if (two) {
-                        term_force(e1 >> kRoleShift, R1, a1, b1, c1, Lx, Ly, Lz, cx, cy, cz);
-                        bx += cx; by += cy; bz += cz;
-                    }
-                }
-            }
-            s_bond[0][lane] = bx;
-            s_bond[1][lane] = by;
-            s_bond[2][lane] = bz;
-            s_nterms[lane] = nt;
+        if (kBondWave && plane == kForcePlanes) {
+            bonded_terms(S, W, pos, vel, s, moving, lane, s_bond, s_nterms);
         } else if (moving) {
             const WorldCtr wc = S.ctr[blk.world];
             const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
@@ -511,11 +518,20 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
             uint32_t codex = 0;
             if (ox < 0) { ox += L0; codex = 2u; } else if (ox >= L0) { ox -= L0; codex = 1u; }
             const float sx = shift_of(codex, Lx);
-            const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz < L2 - 1 ? cz + 1 : L2 - 1;
             const bool pre = cz == 0, post = cz == L2 - 1;
             const int zi = pre ? L2 - 1 : 0;                 // the image cell, if any
             const float szA = pre ? -Lz : 0.0f, szB = post ? Lz : 0.0f;
             const uint32_t zcA = pre ? (2u << 30) : 0u, zcB = post ? (1u << 30) : 0u;
+            // Cells wholly beyond the cutoff are skipped: the gap from the home
+            // bead to a neighbour cell is the distance to the shared face, edge
+            // or corner (fractions in the home cell are exact in FP32). A cell is
+            // skipped only when its squared gap exceeds 1 by a margin far above
+            // the rounding of the pair test, so no candidate the test would
+            // accept is lost and the forces are unchanged.
+            constexpr float kReach = 1.0f + 1e-5f;
+            const float fxr = hp.x - (float)cx, fyr = hp.y - (float)cy, fzr = hp.z - (float)cz;
+            const float gx = plane == 0 ? fxr : (plane == 2 ? 1.0f - fxr : 0.0f);
+            const float gx2 = gx * gx, glo = fzr * fzr, ghi = (1.0f - fzr) * (1.0f - fzr);
             uint32_t ra[3], na[3], rb[3], nb[3], cxy[3];
 #pragma unroll
             for (int r = 0; r < 3; r++) {
@@ -523,10 +539,17 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
                 uint32_t codey = 0;
                 if (oy < 0) { oy += L1; codey = 2u; } else if (oy >= L1) { oy -= L1; codey = 1u; }
                 cxy[r] = (codex << 26) | (codey << 28);
+                const float gy = r == 0 ? fyr : (r == 2 ? 1.0f - fyr : 0.0f);
+                const float g = fmaf(gy, gy, gx2);
+                const bool rowon = g < kReach, loon = rowon && g + glo < kReach, hion = rowon && g + ghi < kReach;
+                const int m0 = pre ? 0 : (loon ? cz - 1 : cz);    // the contiguous run of cells
+                const int m1 = post ? cz : (hion ? cz + 1 : cz);
+                const bool img = pre ? loon : (post && hion);    // the image cell taken
                 const uint32_t row = W->cell_off + (uint32_t)((ox * L1 + oy) * L2);
-                const uint32_t mb = cs[row + zlo], me = cs[row + zhi + 1];
-                const uint32_t ib = (pre || post) ? cs[row + zi] : 0u;
-                const uint32_t ie = (pre || post) ? cs[row + zi + 1] : 0u;
+                const uint32_t mb = cs[row + m0];
+                const uint32_t me = rowon ? cs[row + m1 + 1] : mb;
+                const uint32_t ib = img ? cs[row + zi] : 0u;
+                const uint32_t ie = img ? cs[row + zi + 1] : 0u;
                 ra[r] = pre ? ib : mb;
                 na[r] = pre ? ie - ib : me - mb;
                 rb[r] = pre ? mb : ib;
@@ -541,6 +564,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
                 const float ox_ = op.x + shift_of((e >> 26) & 3u, Lx);
                 const float oy_ = op.y + shift_of((e >> 28) & 3u, Ly);
                 const float oz_ = op.z + shift_of((e >> 30) & 3u, Lz);
+                // (e: periodic-image code of the candidate, bits 26..31)
                 const float dx = hp.x - ox_, dy = hp.y - oy_, dz = hp.z - oz_;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 const float rr = sqrtf(r2);
@@ -558,24 +582,32 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
                 ay = fmaf(ey, F, ay);
                 az = fmaf(ez, F, az);
             };
+            // a position in row r's stream -> its slot and image code
+            auto slot_of = [&](int r, uint32_t q) { return q < na[r] ? ra[r] + q : rb[r] + (q - na[r]); };
+            auto code_of = [&](int r, uint32_t q) { return cxy[r] | (q < na[r] ? zcA : zcB); };
             // evaluate row r's buffered pairs in order, two in flight
             auto flush = [&](int r) {
-                uint32_t q = 0;
-                for (; q + 2 <= cnt[r]; q += 2) {
-                    const uint32_t e0 = s_list[r][q][tid], e1 = s_list[r][q + 1][tid];
-                    const float4 op0 = pos[e0 & kSlotMask], ov0 = vel[e0 & kSlotMask];
-                    const float4 op1 = pos[e1 & kSlotMask], ov1 = vel[e1 & kSlotMask];
-                    pair(e0, op0, ov0, px[r], py[r], pz[r]);
-                    pair(e1, op1, ov1, px[r], py[r], pz[r]);
+                uint32_t c = 0;
+                for (; c + 2 <= cnt[r]; c += 2) {
+                    const uint32_t q0 = s_list[r][c][tid], q1 = s_list[r][c + 1][tid];
+                    const uint32_t m0 = slot_of(r, q0), m1 = slot_of(r, q1);
+                    const float4 op0 = pos[m0], ov0 = vel[m0];
+                    const float4 op1 = pos[m1], ov1 = vel[m1];
+                    pair(code_of(r, q0), op0, ov0, px[r], py[r], pz[r]);
+                    pair(code_of(r, q1), op1, ov1, px[r], py[r], pz[r]);
                 }
-                if (q < cnt[r]) {
-                    const uint32_t e = s_list[r][q][tid];
-                    pair(e, pos[e & kSlotMask], vel[e & kSlotMask], px[r], py[r], pz[r]);
+                if (c < cnt[r]) {
+                    const uint32_t q = s_list[r][c][tid], m = slot_of(r, q);
+                    pair(code_of(r, q), pos[m], vel[m], px[r], py[r], pz[r]);
                 }
                 cnt[r] = 0;
             };
             const uint32_t len0 = na[0] + nb[0], len1 = na[1] + nb[1], len2 = na[2] + nb[2];
-            const uint32_t maxlen = max(len0, max(len1, len2));
+            uint32_t maxlen = max(len0, max(len1, len2));
+            if (maxlen > 0xffffu) {           // stream positions are kept in 16 bits
+                atomicOr(S.flags, PDPD_FLAG_DENSE_ROW);
+                maxlen = 0;
+            }
             for (uint32_t k = 0; k < maxlen; k += kScanWidth) {
 #pragma unroll
                 for (int r = 0; r < 3; r++)
@@ -604,8 +636,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
                         const float dx = hp.x - (p[r][j].x + sx), dy = hp.y - (p[r][j].y + sy),
                                     dz = hp.z - (p[r][j].z + ozs);
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        if (r2 < 1.0f && r2 >= kMinR2 && slot[r][j] != s)
-                            s_list[r][cnt[r]++][tid] = slot[r][j] | cxy[r] | (inA ? zcA : zcB);
+                        if (r2 < 1.0f && r2 >= kMinR2 && slot[r][j] != s) s_list[r][cnt[r]++][tid] = (uint16_t)q;
                     }
                 }
             }
@@ -615,6 +646,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(k
             fy = (py[0] + py[1]) + py[2];
             fz = (pz[0] + pz[1]) + pz[2];
         }
+        if (!kBondWave && plane == 1) bonded_terms(S, W, pos, vel, s, moving, lane, s_bond, s_nterms);
         // combine the three planes in order: (p0 + p1) + p2
         if (plane > 0 && plane < kForcePlanes) {
             s_part[plane - 1][0][lane] = fx;
@@ -700,10 +732,12 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     rec(4);
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     if (fg) {
+        constexpr uint32_t t3 = kForceBeads * kForcePlanes;
         switch (L.force_variant) {
-        case 1: k_force<3, 4><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        case 2: k_force<2, 4><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        default: k_force<2, 5><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        case 1: k_force<3, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        case 2: k_force<2, 5, false><<<fg, t3, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        case 3: k_force<1, 6, false><<<fg, t3, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
+        default: k_force<2, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
         }
     }
     rec(5);

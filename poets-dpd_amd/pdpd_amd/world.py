@@ -256,6 +256,27 @@ def make_world(L, density=3.0, bonded_fraction=0.0, seed=1, a_matrix=None, gamma
                  polymer_id=pid, polymer_offset=poff)
 
 
+def make_lattice_world(L=4, spacing=0.5, nudge=0.3, seed=3):
+    """Monomers on a cubic lattice of the given spacing (density 8 at 0.5), a
+    fraction `nudge` of them moved up by one FP32 ulp per axis: many pairs at
+    exactly r = 1 and beads on cell faces, edges and corners, where the pair
+    test and neighbour-cell culling decide at the boundary."""
+    b = WorldBuilder((L, L, L), dt=0.01, seed=seed)
+    b.add_bead_type("A")
+    b.set_interaction("A", "A", 25.0, 4.5)
+    m = b.add_polymer_type("A", ["A"])
+    rng = np.random.default_rng(seed)
+    g = np.arange(int(round(L / spacing))) * spacing
+    for x in g:
+        for y in g:
+            for z in g:
+                p = np.array([x, y, z])
+                if rng.random() < nudge:
+                    p = np.nextafter(p.astype(np.float32), np.float32(L)).astype(np.float64)
+                b.add_polymer(m, p, rng.standard_normal(3) * 0.1)
+    return b.build()
+
+
 def add_wall(w, thickness=1.0, a_wall=25.0, gamma_wall=4.5):
     """Turn the monomers with x < thickness into a new stationary bead type "S"
     (BeadType::stationary; a wall): velocities set to zero, interactions of S

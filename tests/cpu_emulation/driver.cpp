@@ -83,6 +83,8 @@ int main(int argc, char **argv)
         {"poly_graph", {5, 6, 7}, 0.2, false, 0.0, 4, "graph_steps=2"},
         {"poly_grid", {6, 7, 8}, 0.1, false, 0.0, 2, "force_grid=5"},
         {"poly_v1", {6, 7, 8}, 0.1, false, 0.0, 2, "force_variant=1"},
+        {"wall_v2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=2"},
+        {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
     };
     for (auto &c : cases) {
         fprintf(stderr, "case %s\n", c.name);

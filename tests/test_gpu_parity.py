@@ -49,6 +49,7 @@ def small_worlds():
         # stationary wall slab + chains with theta0 = 1 rad
         "wall_bent8": W.add_wall(W.make_world((8, 8, 8), bonded_fraction=0.2, seed=15, theta0=1.0,
                                               a_matrix=W.random_a_matrix(rngA))),
+        "lattice4": W.make_lattice_world(),
     }
 
 

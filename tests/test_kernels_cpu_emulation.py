@@ -50,9 +50,11 @@ worlds = {
     "poly_6x7x8": W.make_world((6, 7, 8), bonded_fraction=0.2, seed=12, a_matrix=W.random_a_matrix(rngA)),
     "wall_bent6": W.add_wall(W.make_world((6, 6, 6), bonded_fraction=0.2, seed=15, theta0=1.0,
                                           a_matrix=W.random_a_matrix(rngA))),
+    "lattice4": W.make_lattice_world(),
 }
-for name, w in worlds.items():
+for (name, w), var in zip(worlds.items(), (0, 1, 2, 3)):
     ctx = capi.Context(0)
+    ctx.set_config("force_variant", var)
     slot = ctx.add(w)
     ctx.run(1)
     x, ww, f, t = ctx.download_f32(slot)
@@ -71,4 +73,4 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 3, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 4, r.stdout
