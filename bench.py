@@ -5,13 +5,16 @@ density 3, 98,304 beads, 10% of beads in H2T4 chains), one GPU.
 bead-steps/s = beads x steps / wall time of the timed run, state resident in
 HBM (the reference's benchmark_engine.cpp:128-147 rule with Attach excluded).
 Prints ONE JSON line. Extra keys:
-  roofline      pair-force kernel: algorithmic bytes / its average duration
-                (HIP events on the engine's stream, profiled pass of the same run)
+  roofline      pair-force kernel: algorithmic bytes / its average launch duration
+                (HIP events on the engine's stream around back-to-back launches
+                of the last timed step's pair-force kernel)
   cpu_baseline  the reference's NaiveDPDEngineHalfMerge compiled from its
                 sources (-O3 -ffast-math), one thread, bounded sample
   cpu_port_mt   the repository's multi-threaded half-merge restatement
-  batched       (--batch) mixed-size ensemble, one launch set per step vs the
-                same worlds one at a time
+  batched       256-world mixed-size ensemble (8..32 cells per side), one
+                launch set per step, vs a sample of the same worlds one at a time
+  served        served sweep: Poisson arrivals, continuous batching, p50/p99
+                latency and aggregate bead-steps/s
 """
 import argparse
 import json
@@ -123,6 +126,33 @@ def run_batched(args, ctx_cls, W):
             "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "tuning": tune}
 
 
+def run_served(args, capi, W):
+    """Served sweep: Poisson arrivals (50/s), cubic sides log-uniform in
+    [8, 48] cells, 1k-20k steps, per-request a_ij and bonded fraction, interval
+    callbacks every 500 steps, admission by an HBM budget. Continuous batching:
+    arrivals join the running launches at interval boundaries."""
+    reqs = W.served_requests(args.serve, seed=1)
+    srv = capi.Server(device=0, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), interval_steps=500,
+                      time_scale=1.0)
+    seen = [0]
+
+    def on_interval(rid, done, t, x, v, f):
+        seen[0] += 1
+        return True
+    srv.set_callback(on_interval, copy=False)
+    for r in reqs:
+        w = W.make_world(r["L"], bonded_fraction=r["bonded_fraction"], seed=r["seed"], a_matrix=r["a_matrix"])
+        srv.submit(w, r["arrival"], r["steps"])
+    srv.run()
+    st = srv.stats()
+    srv.close()
+    return {"requests": len(reqs), "completed": st["completed"], "p50_latency_s": st["p50_latency_s"],
+            "p99_latency_s": st["p99_latency_s"], "mean_latency_s": st["mean_latency_s"],
+            "value": st["bead_steps_per_s"], "unit": "bead-steps/s", "makespan_s": st["makespan_s"],
+            "max_resident": st["max_resident"], "callbacks": seen[0], "interval_steps": 500,
+            "arrival_rate_per_s": 50.0, "hbm_budget_gib": args.serve_budget_gib}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters give the scale from counter KiB to bytes moved."""
@@ -142,12 +172,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--force-reps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=0, help="also run the batched ensemble of this many worlds")
-    ap.add_argument("--batch-steps", type=int, default=500)
-    ap.add_argument("--batch-solo-sample", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=256, help="batched ensemble of this many worlds (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-solo-sample", type=int, default=16)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--serve", type=int, default=64, help="served sweep with this many requests (0 = skip)")
+    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
     ap.add_argument("--side", type=int, default=32)
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
@@ -186,11 +219,12 @@ def main():
     value = n * args.steps / elapsed
     log(f"bench: {value:.4e} bead-steps/s ({elapsed * 1e3 / args.steps:.4f} ms/step)")
 
+    # the dominant kernel's average launch duration: HIP events on the engine's
+    # stream around back-to-back launches of the last step's pair-force kernel
+    force_ms = ctx.time_force(args.force_reps)
     ctx.run_profiled(args.profile_steps)
     st = ctx.stats()
-    force_ms = st["k_force_ms"]
-    step_ms = sum(st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms", "k_reorder_ms",
-                                  "k_force_ms", "k_stationary_ms"))
+    step_ms = elapsed * 1e3 / args.steps          # whole step, timed region
     achieved = n * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -213,6 +247,9 @@ def main():
     if args.batch:
         log(f"bench: batched ensemble of {args.batch} worlds")
         out["batched"] = run_batched(args, capi.Context, W)
+    if args.serve:
+        log(f"bench: served sweep of {args.serve} requests")
+        out["served"] = run_served(args, capi, W)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         out.update(cpu_baseline(world, args.cpu_seconds))

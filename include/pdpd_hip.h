@@ -173,6 +173,14 @@ int pdpd_get_stats(pdpd_ctx *ctx, pdpd_stats *out);
  * context's stream; fills the k_*_ms averages of pdpd_stats. Synchronous. */
 int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps);
 
+/* Measurement hook (no reference counterpart; the reference times whole Run
+ * calls, benchmark_engine.cpp:128-147): re-launches the pair-force kernel of
+ * the last step reps times back to back on the context's stream, between two
+ * HIP events, and returns the average duration of one launch. The kernel
+ * rewrites the same forces from the same state, so the state is unchanged.
+ * Needs at least one completed step. Synchronous. */
+int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms);
+
 /* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
  * the worlds do not advance); the choice stays in the configuration. */

@@ -897,6 +897,30 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps)
     return check_flags(ctx);
 }
 
+int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms)
+{
+    if (!ctx || !avg_ms) return fail(PDPD_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->dirty || ctx->n_worlds == 0 || ctx->stats.steps == 0)
+        return fail(PDPD_ERR_STATE, "time_force needs a completed step on the current layout");
+    reps = std::max<uint32_t>(1, reps);
+    const StepLaunch L = ctx->step_launch(ctx->cur ^ 1);   // the step that produced the current buffers
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(launch_force(L, ctx->stream));                  // warm
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    for (uint32_t r = 0; r < reps; r++) HIP_TRY(launch_force(L, ctx->stream));
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return check_flags(ctx);
+}
+
 // Times launch shapes that do not change results (the pair-force grid and the
 // graph length) on a saved copy of the state, keeps the fastest.
 int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len)

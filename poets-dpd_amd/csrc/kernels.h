@@ -29,6 +29,8 @@ struct KernelEvents {
 
 uint32_t scan_tiles(uint32_t n_cells);
 hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
+// The pair-force launch of the step L describes (reads and writes buffer L.cur ^ 1).
+hipError_t launch_force(const StepLaunch &L, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
 hipError_t launch_debug_rng(uint32_t n, const uint64_t *th, const uint32_t *a, const uint32_t *b, uint32_t *u32,

@@ -730,20 +730,30 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     rec(3);
     k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
     rec(4);
-    const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
-    if (fg) {
-        constexpr uint32_t t3 = kForceBeads * kForcePlanes;
-        switch (L.force_variant) {
-        case 1: k_force<3, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        case 2: k_force<2, 5, false><<<fg, t3, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        case 3: k_force<1, 6, false><<<fg, t3, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        default: k_force<2, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, cur ^ 1, L.n_force_blocks); break;
-        }
-    }
+    const hipError_t e = launch_force(L, st);
+    if (e != hipSuccess) return e;
     rec(5);
     if (L.n_stationary)
         k_stationary<<<ceil_div(L.n_stationary, kBeadThreads), kBeadThreads, 0, st>>>(Sp, cur ^ 1, L.n_stationary);
     rec(6);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const StepLaunch &L, hipStream_t st)
+{
+    DeviceState Sp = L.S;
+    Sp.ctr = L.S.ctr + L.cur * L.n_worlds;       // counters read by that step
+    const int nxt = L.cur ^ 1;
+    const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
+    if (fg) {
+        constexpr uint32_t t3 = kForceBeads * kForcePlanes;
+        switch (L.force_variant) {
+        case 1: k_force<3, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 2: k_force<2, 5, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 3: k_force<1, 6, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        default: k_force<2, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        }
+    }
     return hipGetLastError();
 }
 

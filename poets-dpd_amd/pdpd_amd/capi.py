@@ -119,6 +119,7 @@ SIGNATURES = {
     "pdpd_server_result": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestResult), C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     "pdpd_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "pdpd_time_force": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -327,6 +328,12 @@ class Context:
         check(self.lib.pdpd_get_stats(self.h, C.byref(s)))
         return s.as_dict()
 
+    def time_force(self, reps=100):
+        """Average duration (ms) of one pair-force launch, HIP events on the engine's stream."""
+        ms = C.c_double()
+        check(self.lib.pdpd_time_force(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
     def autotune(self, steps_per_trial=20):
         buf = C.create_string_buffer(256)
         check(self.lib.pdpd_autotune(self.h, int(steps_per_trial), buf, 256))
@@ -359,8 +366,9 @@ class Server:
     def set_config(self, key, value):
         check(self.lib.pdpd_server_set_config(self.h, key.encode(), str(value).encode()))
 
-    def set_callback(self, fn):
-        """fn(request_id, steps_done, t, x, v, f) -> bool (False cancels)."""
+    def set_callback(self, fn, copy=True):
+        """fn(request_id, steps_done, t, x, v, f) -> bool (False cancels). With
+        copy=False the arrays are views valid only during the call."""
         if fn is None:
             self._cb = None
             check(self.lib.pdpd_server_set_callback(self.h, INTERVAL_FN(), None))
@@ -368,7 +376,8 @@ class Server:
 
         def tramp(user, rid, done, t, n, x, v, f):
             def arr(p):
-                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n, 3)).copy()
+                a = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n, 3))
+                return a.copy() if copy else a
             return 1 if fn(int(rid), int(done), int(t), arr(x), arr(v), arr(f)) else 0
         self._cb = INTERVAL_FN(tramp)
         check(self.lib.pdpd_server_set_callback(self.h, self._cb, None))

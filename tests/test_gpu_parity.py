@@ -257,3 +257,23 @@ def test_unsupported_world_reason(gpu_lib):
         ctx.add(w)
     assert e.value.code == -2
     ctx.close()
+
+
+def test_force_timing_hook_leaves_state_unchanged(gpu_lib):
+    """pdpd_time_force re-launches the last step's pair-force kernel; the state
+    and the trajectory that follows are bit-identical to a run without it."""
+    w = small_worlds()["poly_6x7x8"]
+    _say("time_force idempotence: poly_6x7x8")
+    a, sa = run_gpu(w, 3)
+    before = a.download_f32(sa)
+    ms = a.time_force(5)
+    after = a.download_f32(sa)
+    assert ms > 0
+    for u, v in zip(before[:3], after[:3]):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    a.run(2)
+    b, sb = run_gpu(w, 5)
+    for u, v in zip(a.download_f32(sa)[:3], b.download_f32(sb)[:3]):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    a.close()
+    b.close()
