@@ -134,6 +134,7 @@ struct pdpd_ctx {
     DevBuf<float4> pos[2], vel[2], frc[2];
     DevBuf<uint32_t> key, rank, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
     DevBuf<unsigned long long> scan_tiles;
+    DevBuf<uint32_t> tile_sum;
     DevBuf<WorldDev> wdev;
     DevBuf<WorldCtr> ctr;
     DevBuf<BeadBlock> bblocks, fblocks;
@@ -149,6 +150,7 @@ struct pdpd_ctx {
     uint32_t graph_steps = 20;
     uint32_t force_grid = 0;
     uint32_t force_variant = 0;
+    uint32_t lookback_tiles = 64;
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -164,6 +166,7 @@ struct pdpd_ctx {
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
         key.release(); rank.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
+        tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); stationary.release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -193,6 +196,8 @@ struct pdpd_ctx {
         L.n_stationary = n_stationary;
         L.force_grid = force_grid;
         L.force_variant = force_variant;
+        L.tile_sum = tile_sum.p;
+        L.lookback_tiles = lookback_tiles;
         return L;
     }
 };
@@ -296,6 +301,7 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_alloc(ctx->cell_count, nc));
     HIP_TRY(dev_alloc(ctx->cell_start, nc + 1));
     HIP_TRY(dev_alloc(ctx->scan_tiles, scan_tiles((uint32_t)nc)));
+    HIP_TRY(dev_alloc(ctx->tile_sum, scan_tiles((uint32_t)nc)));
     if (!ctx->flags.p) {
         HIP_TRY(dev_alloc(ctx->flags, 1));
         HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
@@ -657,6 +663,14 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
         if (b.a >= w.n || b.b >= w.n) return fail(PDPD_ERR_INVALID, "bond bead id out of range");
     for (const auto &a : w.angles)
         if (a.a >= w.n || a.b >= w.n || a.c >= w.n) return fail(PDPD_ERR_INVALID, "angle bead id out of range");
+    // a monomer (BeadHash monomer bit) has no bonds: the engine tracks the slots
+    // of polymer beads only
+    for (const auto &b : w.bonds)
+        if (d->is_monomer[b.a] || d->is_monomer[b.b])
+            return fail(PDPD_ERR_INVALID, "a bead in a bond is marked as a monomer");
+    for (const auto &a : w.angles)
+        if (d->is_monomer[a.a] || d->is_monomer[a.b] || d->is_monomer[a.c])
+            return fail(PDPD_ERR_INVALID, "a bead in an angle bond is marked as a monomer");
 
     int slot = ctx->next_slot++;
     w.slot = slot;
@@ -816,6 +830,13 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         ctx->graph_version = ~0ull;
         return PDPD_OK;
     }
+    if (k == "scan_lookback_tiles") {
+        uint32_t n;
+        if (parse_u32(v, 0, 1u << 20, &n)) return fail(PDPD_ERR_INVALID, "scan_lookback_tiles must be an integer >= 0");
+        ctx->lookback_tiles = n;
+        ctx->graph_version = ~0ull;
+        return PDPD_OK;
+    }
     if (k == "force_variant") {
         uint32_t n;
         if (parse_u32(v, 0, kForceVariants - 1, &n))
@@ -834,6 +855,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     if (k == "graph_steps") v = std::to_string(ctx->graph_steps);
     else if (k == "force_grid") v = std::to_string(ctx->force_grid);
     else if (k == "force_variant") v = std::to_string(ctx->force_variant);
+    else if (k == "scan_lookback_tiles") v = std::to_string(ctx->lookback_tiles);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -846,6 +868,8 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "graph_steps=" + std::to_string(ctx->graph_steps) + "\tsteps per captured hipGraph (even; 0 = direct launches)\n";
     s += "force_grid=" + std::to_string(ctx->force_grid) +
          "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
+    s += "scan_lookback_tiles=" + std::to_string(ctx->lookback_tiles) +
+         "\tcell scan: single pass with look-back up to this many 2048-cell tiles, three phases above\n";
     s += "force_variant=" + std::to_string(ctx->force_variant) +
          "\tpair-force kernel variant: 0 = 2 loads per row in flight, 5 waves/SIMD, bond wave; 1 = 3, 4, bond wave;"
          " 2 = 2, 5, bonds in the middle-plane wave; 3 = 2, 6, bonds in the middle-plane wave\n";

@@ -17,6 +17,8 @@ struct StepLaunch {
     uint32_t n_stationary;   // beads of stationary types
     uint32_t force_grid;     // 0 = one workgroup per force block, else persistent grid size
     uint32_t force_variant;  // pair-force kernel variant (loads in flight / occupancy), same results
+    uint32_t *tile_sum;      // scan_tiles(n_cells) entries: tile sums of the three-phase scan
+    uint32_t lookback_tiles; // single-pass look-back scan up to this many tiles, three-phase above
 };
 
 constexpr uint32_t kForceVariants = 4;

@@ -296,6 +296,78 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ co
     }
 }
 
+// Large layouts (more tiles than StepLaunch::lookback_tiles): reduce each tile,
+// scan the tile sums in one workgroup, then rescan each tile from its offset.
+// Three launches, no inter-workgroup waiting.
+constexpr int kTopThreads = 1024, kTopPerThread = 4;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ count, uint32_t n,
+                                                             uint32_t *__restrict__ tile_sum)
+{
+    __shared__ uint32_t s_wsum[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; k++) {
+        const uint32_t i = base + k;
+        acc += (i < n) ? count[i] : 0u;
+    }
+    uint32_t tot;
+    (void)block_scan<kScanThreads / 64>(acc, s_wsum, &tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kTopThreads) void k_scan_top(uint32_t *__restrict__ tile_sum, uint32_t n_tiles,
+                                                         uint32_t *__restrict__ cell_start, uint32_t n)
+{
+    __shared__ uint32_t s_wsum[kTopThreads / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n_tiles; base += kTopThreads * kTopPerThread) {   // uniform loop
+        const uint32_t i0 = base + threadIdx.x * kTopPerThread;
+        uint32_t v[kTopPerThread], acc = 0;
+#pragma unroll
+        for (int k = 0; k < kTopPerThread; k++) {
+            v[k] = (i0 + k < n_tiles) ? tile_sum[i0 + k] : 0u;
+            acc += v[k];
+        }
+        uint32_t tot;
+        uint32_t ex = block_scan<kTopThreads / 64>(acc, s_wsum, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kTopPerThread; k++) {
+            if (i0 + k < n_tiles) tile_sum[i0 + k] = ex;
+            ex += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) cell_start[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict__ count, uint32_t n,
+                                                           uint32_t *__restrict__ cell_start,
+                                                           const uint32_t *__restrict__ tile_off)
+{
+    __shared__ uint32_t s_wsum[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread], acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; k++) {
+        const uint32_t i = base + k;
+        v[k] = (i < n) ? count[i] : 0u;
+        acc += v[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_scan<kScanThreads / 64>(acc, s_wsum, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; k++) {
+        const uint32_t i = base + k;
+        if (i < n) {
+            cell_start[i] = ex;
+            count[i] = 0u;          // ready for the next step's arrivals
+        }
+        ex += v[k];
+    }
+}
+
 // ------------------------------------------------------- sort scatter ----
 
 __global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur, uint32_t n)
@@ -310,7 +382,10 @@ __global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur
 
 // Stable order inside a cell: beads ranked by global bead index, so the order
 // (and every force sum) is independent of arrival order and of the batch.
-__global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur, uint32_t n)
+// slot_of_bead is read only for bonded beads (the bonded terms) and stationary
+// ones; monomers (BeadHash bit 27) have no bonds, so their entries are not
+// written unless stationary beads are resident.
+__global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur, uint32_t n, int all_slots)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -323,10 +398,11 @@ __global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur
     uint32_t rank = 0;
     for (uint32_t q = a; q < b; q++) rank += S.sorted_idx[q] < me ? 1u : 0u;
     const uint32_t dst = a + rank;
-    S.pos[nxt][dst] = S.pos[cur][s];
+    const float4 p = S.pos[cur][s];
+    S.pos[nxt][dst] = p;
     S.vel[nxt][dst] = S.vel[cur][s];
     if (e & kInactive) S.frc[nxt][dst] = S.frc[cur][s];
-    S.slot_of_bead[me] = dst;
+    if (all_slots || !((fbits(p.w) >> 27) & 1u)) S.slot_of_bead[me] = dst;
 }
 
 // -------------------------------------------------------------- bonds ----
@@ -722,13 +798,19 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     k_integrate<<<ig, kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks, S.ctr + (par ^ 1) * L.n_worlds,
                                               L.n_worlds);
     rec(1);
-    k_scan<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, S.scan_tiles, S.scan_ticket, nt,
-                                         S.flags);
+    if (nt <= L.lookback_tiles) {
+        k_scan<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, S.scan_tiles, S.scan_ticket, nt,
+                                             S.flags);
+    } else {
+        k_scan_reduce<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, L.tile_sum);
+        k_scan_top<<<1, kTopThreads, 0, st>>>(L.tile_sum, nt, S.cell_start, L.n_cells);
+        k_scan_down<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, L.tile_sum);
+    }
     rec(2);
     const uint32_t gb = ceil_div(L.n_beads, kBeadThreads);
     k_scatter<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
     rec(3);
-    k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
+    k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads, L.n_stationary > 0 ? 1 : 0);
     rec(4);
     const hipError_t e = launch_force(L, st);
     if (e != hipSuccess) return e;

@@ -85,6 +85,7 @@ int main(int argc, char **argv)
         {"poly_v1", {6, 7, 8}, 0.1, false, 0.0, 2, "force_variant=1"},
         {"wall_v2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=2"},
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
+        {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
     };
     for (auto &c : cases) {
         fprintf(stderr, "case %s\n", c.name);

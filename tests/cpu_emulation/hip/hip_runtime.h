@@ -96,6 +96,7 @@ inline void resolve_waves()
             const int lane = l - w0;
             int src = lane;
             if (op == 0) { f.res = ballot; f.wait = 0; continue; }
+            if (op == 4) { f.res = 0; f.wait = 0; continue; }
             if (op == 1) src = lane - f.arg < 0 ? lane : lane - f.arg;     // shfl_up
             if (op == 2) src = f.arg & 63;                                 // shfl
             if (op == 3) src = lane ^ f.arg;                               // shfl_xor
@@ -170,6 +171,7 @@ inline uint32_t __shfl_up(uint32_t v, unsigned d, int = 64) { return (uint32_t)e
 inline uint32_t __shfl(uint32_t v, int src, int = 64) { return (uint32_t)emu::wave_op(2, v, src); }
 inline uint32_t __shfl_xor(uint32_t v, int m, int = 64) { return (uint32_t)emu::wave_op(3, v, m); }
 inline void __builtin_amdgcn_s_sleep(int) { emu::yield_(); }
+inline void __builtin_amdgcn_wave_barrier() { (void)emu::wave_op(4, 0, 0); }
 template <class T, class U> inline T atomicAdd(T *p, U v) { T o = *p; *p = o + (T)v; return o; }
 template <class T, class U> inline T atomicOr(T *p, U v) { T o = *p; *p = o | (T)v; return o; }
 template <class T> inline T __hip_atomic_load(T *p, int, int) { return *p; }

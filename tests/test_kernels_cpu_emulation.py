@@ -55,6 +55,7 @@ worlds = {
 for (name, w), var in zip(worlds.items(), (0, 1, 2, 3)):
     ctx = capi.Context(0)
     ctx.set_config("force_variant", var)
+    ctx.set_config("scan_lookback_tiles", (var & 1) * 64)     # both cell-scan forms
     slot = ctx.add(w)
     ctx.run(1)
     x, ww, f, t = ctx.download_f32(slot)
