@@ -97,7 +97,7 @@ def run_batched(args, ctx_cls, W):
     ctx = ctx_cls(0)
     for w in worlds:
         ctx.add(w)
-    tune = ctx.autotune(10) if args.autotune else "heuristic"
+    tune = ctx.autotune(10) if not args.no_batch_autotune else "heuristic"
     ctx.run(2)
     ctx.sync()
     t0 = time.perf_counter()
@@ -178,7 +178,9 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="batched ensemble of this many worlds (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-solo-sample", type=int, default=16)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="autotune the plain engine's launch shape too")
+    ap.add_argument("--no-batch-autotune", action="store_true",
+                    help="batched ensemble with the default launch shape instead of the autotuner's")
     ap.add_argument("--serve", type=int, default=64, help="served sweep with this many requests (0 = skip)")
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
     ap.add_argument("--side", type=int, default=32)
