@@ -54,6 +54,7 @@ class World:
     polymer_id: np.ndarray         # (n,) uint32
     polymer_offset: np.ndarray     # (n,) uint8
     stationary: np.ndarray = None  # (nt,) uint8 or None
+    origin: np.ndarray = None      # (3,) offset of the box corner in files, or None for 0
 
     @property
     def n_beads(self):
