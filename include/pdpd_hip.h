@@ -43,7 +43,7 @@ enum {
 /* Error flags a step can raise (pdpd_stats.flags). */
 #define PDPD_FLAG_BAD_POSITION   0x1u   /* non-finite x, or a jump of more than one box */
 #define PDPD_FLAG_SCAN_TIMEOUT   0x2u   /* the cell scan look-back ran out of spins      */
-#define PDPD_FLAG_DENSE_ROW      0x4u   /* 3 neighbouring cells hold more than 65535 beads */
+#define PDPD_FLAG_DENSE_ROW      0x4u   /* reserved: no longer raised (rows of any length are handled) */
 
 /* Text of the last error on this thread ("" if none). */
 const char *pdpd_last_error(void);

@@ -140,9 +140,10 @@ struct pdpd_ctx {
     DevBuf<BeadBlock> bblocks, fblocks;
     DevBuf<uint32_t> term_start, bead_terms;
     DevBuf<TermRec> terms;
+    DevBuf<double> term_frc;
     DevBuf<StationaryBead> stationary;
     int cur = 0;
-    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0;
+    uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0, n_terms = 0;
     std::vector<int> order;               // slots in layout order
     std::vector<WorldCtr> ctr_host;
 
@@ -168,7 +169,7 @@ struct pdpd_ctx {
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
-        bead_terms.release(); terms.release(); stationary.release();
+        bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -181,7 +182,7 @@ struct pdpd_ctx {
         S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
         S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
         S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
-        S.term_start = term_start.p; S.bead_terms = bead_terms.p; S.terms = terms.p;
+        S.term_start = term_start.p; S.bead_terms = bead_terms.p; S.terms = terms.p; S.term_frc = term_frc.p;
         S.stationary_beads = stationary.p;
         return S;
     }
@@ -194,6 +195,7 @@ struct pdpd_ctx {
         L.n_beads = n_beads; L.n_cells = n_cells; L.n_worlds = n_worlds;
         L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks;
         L.n_stationary = n_stationary;
+        L.n_terms = n_terms;
         L.force_grid = force_grid;
         L.force_variant = force_variant;
         L.tile_sum = tile_sum.p;
@@ -342,6 +344,7 @@ int relayout(pdpd_ctx *ctx)
     // in the order the per-polymer loop applies them: its bonds, then its angles
     std::vector<TermRec> trec;
     std::vector<uint32_t> tstart(nb + 1, 0), terms;
+    if (ws.size() >= (1u << (32 - kTermWorldShift))) return fail(PDPD_ERR_INVALID, "too many worlds in one context");
     for (size_t i = 0; i < ws.size(); i++) {
         const WorldHost &w = *ws[i];
         if (w.bonds.empty() && w.angles.empty()) continue;
@@ -351,6 +354,7 @@ int relayout(pdpd_ctx *ctx)
             for (const auto &rb : roles) per[rb.second].push_back((rb.first << kRoleShift) | k);
             TermRec r = r0;
             r.a += w.bead_off; r.b += w.bead_off; r.c += w.bead_off;
+            r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
             trec.push_back(r);
         };
         for (const TermRec &r : w.bonds) add(r, {{kRoleBondHead, r.a}, {kRoleBondTail, r.b}});
@@ -364,13 +368,14 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_upload(ctx->terms, trec));
     HIP_TRY(dev_upload(ctx->term_start, tstart));
     HIP_TRY(dev_upload(ctx->bead_terms, terms));
+    ctx->n_terms = (uint32_t)trec.size();
+    HIP_TRY(dev_alloc(ctx->term_frc, std::max<size_t>(1, trec.size()) * kTermFrcStride));
 
     std::vector<StationaryBead> sb;
     for (size_t i = 0; i < ws.size(); i++)
         for (uint32_t b : ws[i]->stationary_beads) sb.push_back({ws[i]->bead_off + b, (int32_t)i});
     ctx->n_stationary = (uint32_t)sb.size();
     HIP_TRY(dev_upload(ctx->stationary, sb));
-    HIP_TRY(dev_upload(ctx->bead_terms, terms));
 
     ctx->stats.n_worlds = ctx->n_worlds;
     ctx->stats.n_beads = ctx->n_beads;
@@ -482,8 +487,6 @@ int check_flags(pdpd_ctx *ctx)
     HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (f & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
-    if (f & PDPD_FLAG_DENSE_ROW)
-        return fail(PDPD_ERR_STATE, "three neighbouring cells hold more than 65535 beads; forces were not computed");
     return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
 }
 

@@ -51,9 +51,16 @@ struct BeadBlock {
 // Bonded terms: one record per Hookean bond or angle, global bead indices.
 //   kind 0 (Hookean): a = head, b = tail, p0 = r0
 //   kind 1 (angle):   a = head, b = middle, c = tail, p0 = cos(theta0), p1 = sin(theta0)
-// Bonded terms are evaluated in FP64 (term_force).
+// kind holds the term kind in bits 0-7 and the world (index into the world
+// table) in bits 8-31. Bonded terms are evaluated in FP64 (k_terms), once per
+// term per step, into term_frc.
 struct TermRec { uint32_t a, b, c, kind; double kappa, p0, p1; };
 constexpr uint32_t kTermBond = 0, kTermAngle = 1;
+constexpr uint32_t kTermWorldShift = 8;
+// term_frc: 6 doubles per term. Hookean: the tail's force (the head gets its
+// negation). Angle: the head's force, then the tail's (the middle gets
+// -(head + tail)).
+constexpr uint32_t kTermFrcStride = 6;
 
 // Each bead's terms are bead_terms[term_start[bead], term_start[bead + 1]),
 // (role << 29) | record index, in the order the reference's per-polymer loop
@@ -87,6 +94,7 @@ struct DeviceState {
     uint32_t *term_start;      // n_beads + 1, by global bead index
     uint32_t *bead_terms;
     TermRec *terms;
+    double *term_frc;          // kTermFrcStride per term, written by k_terms each step
     StationaryBead *stationary_beads;
 };
 

@@ -15,6 +15,7 @@ struct StepLaunch {
     uint32_t n_bead_blocks;
     uint32_t n_force_blocks;
     uint32_t n_stationary;   // beads of stationary types
+    uint32_t n_terms;        // bonded terms (Hookean bonds and angles) of all worlds
     uint32_t force_grid;     // 0 = one workgroup per force block, else persistent grid size
     uint32_t force_variant;  // pair-force kernel variant (loads in flight / occupancy), same results
     uint32_t *tile_sum;      // scan_tiles(n_cells) entries: tile sums of the three-phase scan
@@ -31,7 +32,8 @@ struct KernelEvents {
 
 uint32_t scan_tiles(uint32_t n_cells);
 hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
-// The pair-force launch of the step L describes (reads and writes buffer L.cur ^ 1).
+// The force launches of the step L describes (bonded terms, then pair forces;
+// they read and write buffer L.cur ^ 1).
 hipError_t launch_force(const StepLaunch &L, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);

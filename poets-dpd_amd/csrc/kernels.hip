@@ -8,14 +8,17 @@
 //   k_scan        single-pass exclusive scan of the cell counts (look-back)
 //   k_scatter     sorted position of every bead's slot
 //   k_reorder     stable gather into the other buffer, bead-index order in a cell
+//   k_terms       one thread per Hookean bond / angle: its forces in FP64
+//                 (terms of naive_dpd_engine_half_merge.hpp:638-690 /
+//                 dpd_maths_core_half_step.hpp:178-231), once per term
 //   k_force       per workgroup 64 consecutive beads: three waves for the pair
 //                 forces, one per x-plane of the 27-cell stencil, neighbours read
 //                 straight from the cell-sorted arrays; register accumulation, no
 //                 atomics (dpd_maths_core_half_step_raw.hpp:129-216; RNG of
-//                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79); a fourth wave
-//                 evaluates the home beads' Hookean and angle terms in FP64 in
-//                 the per-polymer order of naive_dpd_engine_half_merge.hpp:376-385
-//                 (terms of :638-690 / dpd_maths_core_half_step.hpp:178-231)
+//                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79); the home beads'
+//                 bonded terms summed in FP64 in the per-polymer order of
+//                 naive_dpd_engine_half_merge.hpp:376-385 (by a fourth wave or
+//                 after the middle plane's pairs)
 //   k_stationary  (only with stationary bead types) v = 0 after the pair forces
 // The final half-kick v += f dt/2 of a step is folded into the next k_integrate
 // (the engine stores w = v - f dt/2).
@@ -414,33 +417,42 @@ __device__ __forceinline__ double min_image(double d, double len)
     return d;
 }
 
-// Contribution of one bonded term to one of its beads, in FP64 from the FP32
-// positions (differences of floats are exact in double). FP64 keeps the
-// theta0 != 0 angle force, which divides by sin(phi), stable near straight
-// chains. Hookean: naive_dpd_engine_half_merge.hpp:638-654; angle:
+// One bonded term per thread, in FP64 from the FP32 positions (differences of
+// floats are exact in double). FP64 keeps the theta0 != 0 angle force, which
+// divides by sin(phi), stable near straight chains. Hookean:
+// naive_dpd_engine_half_merge.hpp:638-654 (stored: the tail's force
+// dx * kappa (r0 - r) / r; the head gets its negation); angle:
 // dpd_maths_core_half_step.hpp:178-231 with the prefactor floor of
-// naive_dpd_engine.hpp:681-682 (the argument clamped at 0).
-__device__ __forceinline__ void term_force(uint32_t role, const TermRec &R, const float3 &xa, const float3 &xb,
-                                           const float3 &xc, double Lx, double Ly, double Lz, double &cx, double &cy,
-                                           double &cz)
+// naive_dpd_engine.hpp:681-682, the argument clamped at 0 (stored: head, then
+// tail; the middle gets -(head + tail)). Terms of frozen worlds are skipped.
+__global__ __launch_bounds__(kBeadThreads) void k_terms(DeviceState S, int nxt, uint32_t n_terms)
 {
-    if (role <= kRoleBondTail) {
-        const double dx = min_image((double)xb.x - (double)xa.x, Lx);
-        const double dy = min_image((double)xb.y - (double)xa.y, Ly);
-        const double dz = min_image((double)xb.z - (double)xa.z, Lz);
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_terms) return;
+    const TermRec R = S.terms[k];
+    const uint32_t w = R.kind >> kTermWorldShift;
+    if (S.ctr[w].remaining == 0) return;
+    const float4 *__restrict__ pos = S.pos[nxt];
+    const double Lx = S.worlds[w].Lf[0], Ly = S.worlds[w].Lf[1], Lz = S.worlds[w].Lf[2];
+    double *__restrict__ out = S.term_frc + (size_t)k * kTermFrcStride;
+    const float4 pa = pos[S.slot_of_bead[R.a]], pb = pos[S.slot_of_bead[R.b]];
+    if ((R.kind & 0xffu) == kTermBond) {
+        const double dx = min_image((double)pb.x - (double)pa.x, Lx);
+        const double dy = min_image((double)pb.y - (double)pa.y, Ly);
+        const double dz = min_image((double)pb.z - (double)pa.z, Lz);
         const double r = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
         const double hf = R.kappa * (R.p0 - r);
         const double sc = hf / r;
-        const double sgn = role == kRoleBondHead ? -1.0 : 1.0;   // head -= f, tail += f
-        cx = sgn * (dx * sc);
-        cy = sgn * (dy * sc);
-        cz = sgn * (dz * sc);
+        out[0] = dx * sc;
+        out[1] = dy * sc;
+        out[2] = dz * sc;
         return;
     }
-    const double d0x = min_image((double)xb.x - (double)xa.x, Lx), d0y = min_image((double)xb.y - (double)xa.y, Ly),
-                 d0z = min_image((double)xb.z - (double)xa.z, Lz);
-    const double d1x = min_image((double)xc.x - (double)xb.x, Lx), d1y = min_image((double)xc.y - (double)xb.y, Ly),
-                 d1z = min_image((double)xc.z - (double)xb.z, Lz);
+    const float4 pc = pos[S.slot_of_bead[R.c]];
+    const double d0x = min_image((double)pb.x - (double)pa.x, Lx), d0y = min_image((double)pb.y - (double)pa.y, Ly),
+                 d0z = min_image((double)pb.z - (double)pa.z, Lz);
+    const double d1x = min_image((double)pc.x - (double)pb.x, Lx), d1y = min_image((double)pc.y - (double)pb.y, Ly),
+                 d1z = min_image((double)pc.z - (double)pb.z, Lz);
     const double r01 = sqrt(fma(d0z, d0z, fma(d0y, d0y, d0x * d0x)));
     const double r12 = sqrt(fma(d1z, d1z, fma(d1y, d1y, d1x * d1x)));
     const double mag = r01 * r12;
@@ -458,9 +470,8 @@ __device__ __forceinline__ void term_force(uint32_t role, const TermRec &R, cons
         hx = (d0x * o1 - d1x) * fm; hy = (d0y * o1 - d1y) * fm; hz = (d0z * o1 - d1z) * fm;
         tx = (d0x - d1x * o2) * fm; ty = (d0y - d1y * o2) * fm; tz = (d0z - d1z * o2) * fm;
     }
-    if (role == kRoleAngleHead) { cx = hx; cy = hy; cz = hz; }
-    else if (role == kRoleAngleTail) { cx = tx; cy = ty; cz = tz; }
-    else { cx = -(hx + tx); cy = -(hy + ty); cz = -(hz + tz); }
+    out[0] = hx; out[1] = hy; out[2] = hz;
+    out[3] = tx; out[4] = ty; out[5] = tz;
 }
 
 // --------------------------------------------------------- pair force ----
@@ -472,7 +483,8 @@ __device__ __forceinline__ float shift_of(uint32_t code, float L)
     return code == 1u ? L : (code == 2u ? -L : 0.0f);
 }
 
-constexpr int kRowList = 8;       // accepted candidates buffered per row and lane
+constexpr int kQueue = 256;       // accepted pairs buffered per pair wave before they are evaluated
+constexpr int kLaneList = 16;     // accepted pairs buffered per home bead (queue positions, 8 bits)
 constexpr uint32_t kXcds = 8;     // MI355X: 8 accelerator complex dies, each with its own L2
 
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD i mod 8).
@@ -493,23 +505,11 @@ __device__ __forceinline__ float3 load_xyz(const float4 *__restrict__ p, uint32_
     return make_float3(q[0], q[1], q[2]);
 }
 
-// Pair forces. A workgroup takes 64 consecutive slots of one world (one home
-// bead per lane) and has one wave per x-plane of the 27-cell stencil. A wave
-// walks the three rows (dy = -1, 0, +1) of its plane side by side: the
-// candidates of every row are loaded together, and each row keeps its own list
-// of accepted candidates and its own partial force. Per row the candidates come
-// in z order -1, 0, +1 (an image cell below 0, the contiguous run of cells, an
-// image cell above L-1) and, within a cell, in slot order. The partial sums are
-// combined as ((r0 + r1) + r2) per plane and ((p0 + p1) + p2) over planes, so
-// every force is summed in one fixed order, independent of batch and launch.
-// kScanWidth (candidate loads in flight per row and lane) and the occupancy
-// target trade registers for memory-level parallelism; every variant gives
-// bit-identical forces.
-// The bonded terms of the bead in slot s, summed in FP64 in the per-polymer
-// order into s_bond / s_nterms (lane = the bead's lane in its workgroup).
-__device__ __forceinline__ void bonded_terms(const DeviceState &S, const WorldDev *__restrict__ W,
-                                             const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                             uint32_t s, bool moving, int lane, double (*s_bond)[kForceBeads],
+// The bonded terms of the bead in slot s (computed by k_terms), summed in FP64
+// in the per-polymer order into s_bond / s_nterms (lane = the bead's lane in
+// its workgroup).
+__device__ __forceinline__ void bonded_terms(const DeviceState &S, const float4 *__restrict__ vel, uint32_t s,
+                                             bool moving, int lane, double (*s_bond)[kForceBeads],
                                              uint32_t *s_nterms)
 {
     double bx = 0.0, by = 0.0, bz = 0.0;
@@ -518,14 +518,15 @@ __device__ __forceinline__ void bonded_terms(const DeviceState &S, const WorldDe
         const uint32_t b = fbits(vel[s].w);
         const uint32_t t0 = S.term_start[b], t1 = S.term_start[b + 1];
         nt = t1 - t0;
-        const double Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
         for (uint32_t t = t0; t < t1; t++) {
             const uint32_t e = S.bead_terms[t];
-            const TermRec R = S.terms[e & ((1u << kRoleShift) - 1u)];
-            const float3 xa = load_xyz(pos, S.slot_of_bead[R.a]), xb = load_xyz(pos, S.slot_of_bead[R.b]),
-                         xc = load_xyz(pos, S.slot_of_bead[R.c]);
+            const uint32_t role = e >> kRoleShift;
+            const double *__restrict__ F = S.term_frc + (size_t)(e & ((1u << kRoleShift) - 1u)) * kTermFrcStride;
             double cx, cy, cz;
-            term_force(e >> kRoleShift, R, xa, xb, xc, Lx, Ly, Lz, cx, cy, cz);
+            if (role == kRoleBondHead) { cx = -F[0]; cy = -F[1]; cz = -F[2]; }
+            else if (role == kRoleBondTail || role == kRoleAngleHead) { cx = F[0]; cy = F[1]; cz = F[2]; }
+            else if (role == kRoleAngleTail) { cx = F[3]; cy = F[4]; cz = F[5]; }
+            else { cx = -(F[0] + F[3]); cy = -(F[1] + F[4]); cz = -(F[2] + F[5]); }
             bx += cx; by += cy; bz += cz;
         }
     }
@@ -535,17 +536,52 @@ __device__ __forceinline__ void bonded_terms(const DeviceState &S, const WorldDe
     s_nterms[lane] = nt;
 }
 
-// kBondWave: a fourth wave evaluates the bonded terms beside the pair waves;
-// otherwise the middle plane's wave evaluates them after its pairs (three waves
-// per workgroup, more workgroups resident). Same sums either way.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+    return v;
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pair forces. A workgroup takes 64 consecutive slots of one world (one home
+// bead per lane) and has one wave per x-plane of the 27-cell stencil. A wave
+// scans the three rows (dy = -1, 0, +1) of its plane in turn; per row the
+// candidates come in z order -1, 0, +1 (an image cell below 0, the contiguous
+// run of cells, an image cell above L-1) and, within a cell, in slot order.
+// Accepted candidates of all 64 home beads go to one queue per wave; when the
+// queue fills, all lanes evaluate queued pairs side by side (64 per round,
+// whatever bead they belong to) and store each pair's unit vector and force
+// magnitude; each home bead then accumulates its own pairs in acceptance
+// order. So every force is summed in one fixed order: per row in candidate
+// order, rows as ((r0 + r1) + r2), planes as ((p0 + p1) + p2), independent of
+// batch, launch shape and queue flushes. kScanWidth (candidate loads in flight
+// per lane) and the occupancy target trade registers for memory-level
+// parallelism; every variant gives bit-identical forces.
+// kBondWave: a fourth wave sums the bonded terms beside the pair waves;
+// otherwise the middle plane's wave sums them after its pairs (three waves per
+// workgroup, more workgroups resident). Same sums either way.
 template <int kScanWidth, int kWaves, bool kBondWave>
 __global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
 __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
 {
     constexpr int kThreads = kBondWave ? kForceThreads : kForceBeads * kForcePlanes;
-    // accepted candidates as positions in their row's stream (the slot and the
-    // periodic-image code follow from the row), 16 bits each
-    __shared__ uint16_t s_list[3][kRowList][kForceBeads * kForcePlanes];
+    constexpr int kPairLanes = kForceBeads * kForcePlanes;
+    // per pair wave: queued pairs (candidate slot | image code bits 26..31, and
+    // home lane) and their evaluated (unit vector, force magnitude)
+    __shared__ uint32_t s_qslot[kForcePlanes][kQueue];
+    __shared__ uint8_t s_qhome[kForcePlanes][kQueue];
+    __shared__ float4 s_qres[kForcePlanes][kQueue];
+    // per home bead and plane: its queued pairs in acceptance order
+    __shared__ uint8_t s_list[kLaneList][kPairLanes];
+    __shared__ float4 s_hp[kForceBeads], s_hv[kForceBeads];
     __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
     __shared__ double s_bond[3][kForceBeads];
     __shared__ uint32_t s_nterms[kForceBeads];
@@ -554,6 +590,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
     const int tid = threadIdx.x;
     const int plane = tid / kForceBeads;            // wave-uniform
     const int lane = tid % kForceBeads;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const float4 *__restrict__ pos = S.pos[nxt];
     const float4 *__restrict__ vel = S.vel[nxt];
     const uint32_t *__restrict__ cs = S.cell_start;
@@ -566,26 +603,29 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
             s_a[i] = W->a[i];
             s_sg[i] = W->sg[i];
         }
-        __syncthreads();
         const uint32_t s = blk.begin + lane;
         const bool live = s < blk.end;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        const uint32_t sl = live ? s : blk.begin;            // a valid slot for every lane's loads
+        const float4 hp = pos[sl];
+        const uint32_t hh = fbits(hp.w);
         // a stationary home bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390);
         // its pairs with moving beads are evaluated from the moving side
-        const bool moving = live && !((W->stationary >> (fbits(pos[s].w) >> 28)) & 1u);
+        const bool moving = live && !((W->stationary >> (hh >> 28)) & 1u);
+        if (plane == 0) {
+            s_hp[lane] = hp;
+            s_hv[lane] = vel[sl];
+        }
+        __syncthreads();
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         if (kBondWave && plane == kForcePlanes) {
-            bonded_terms(S, W, pos, vel, s, moving, lane, s_bond, s_nterms);
-        } else if (moving) {
+            bonded_terms(S, vel, s, moving, lane, s_bond, s_nterms);
+        } else {
             const WorldCtr wc = S.ctr[blk.world];
             const uint64_t th = step_hash(wc.t, W->seed_lo, W->seed_hi);
             const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
             const float scale = W->rng_scale;
             const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
             const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
-            const float4 hp = pos[s];
-            const float4 hv = vel[s];
-            const uint32_t hh = fbits(hp.w);
-            const uint32_t trow = (hh >> 28) * kMaxTypes;
             const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
 
             // the plane's three rows; per row a stream of at most two slot ranges,
@@ -617,7 +657,8 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 cxy[r] = (codex << 26) | (codey << 28);
                 const float gy = r == 0 ? fyr : (r == 2 ? 1.0f - fyr : 0.0f);
                 const float g = fmaf(gy, gy, gx2);
-                const bool rowon = g < kReach, loon = rowon && g + glo < kReach, hion = rowon && g + ghi < kReach;
+                const bool rowon = moving && g < kReach, loon = rowon && g + glo < kReach,
+                           hion = rowon && g + ghi < kReach;
                 const int m0 = pre ? 0 : (loon ? cz - 1 : cz);    // the contiguous run of cells
                 const int m1 = post ? cz : (hion ? cz + 1 : cz);
                 const bool img = pre ? loon : (post && hion);    // the image cell taken
@@ -632,97 +673,117 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 nb[r] = pre ? me - mb : ie - ib;
             }
 
+            uint32_t *__restrict__ qslot = s_qslot[plane];
+            uint8_t *__restrict__ qhome = s_qhome[plane];
+            float4 *__restrict__ qres = s_qres[plane];
             float px[3] = {0.0f, 0.0f, 0.0f}, py[3] = {0.0f, 0.0f, 0.0f}, pz[3] = {0.0f, 0.0f, 0.0f};
-            uint32_t cnt[3] = {0u, 0u, 0u};
-            // one accepted pair: dpd_maths_core_half_step_raw.hpp:129-216 with the
-            // terms grouped as a wr + sgw (scale u - sgw rdotv)
-            auto pair = [&](uint32_t e, const float4 &op, const float4 &ov, float &ax, float &ay, float &az) {
-                const float ox_ = op.x + shift_of((e >> 26) & 3u, Lx);
-                const float oy_ = op.y + shift_of((e >> 28) & 3u, Ly);
-                const float oz_ = op.z + shift_of((e >> 30) & 3u, Lz);
-                // (e: periodic-image code of the candidate, bits 26..31)
-                const float dx = hp.x - ox_, dy = hp.y - oy_, dz = hp.z - oz_;
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float rr = sqrtf(r2);
-                const float inv_r = 1.0f / rr;
-                const float wr = 1.0f - rr;
-                const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
-                const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
-                const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                const uint32_t oh = fbits(op.w);
-                const uint32_t ti = trow + (oh >> 28);
-                const float sgw = s_sg[ti] * wr;
-                const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
-                const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
-                ax = fmaf(ex, F, ax);
-                ay = fmaf(ey, F, ay);
-                az = fmaf(ez, F, az);
-            };
-            // a position in row r's stream -> its slot and image code
-            auto slot_of = [&](int r, uint32_t q) { return q < na[r] ? ra[r] + q : rb[r] + (q - na[r]); };
-            auto code_of = [&](int r, uint32_t q) { return cxy[r] | (q < na[r] ? zcA : zcB); };
-            // evaluate row r's buffered pairs in order, two in flight
-            auto flush = [&](int r) {
+            uint32_t cnt[3] = {0u, 0u, 0u};     // this lane's queued pairs per row
+            uint32_t queued = 0;                // queue fill, the same in every lane
+
+            // Evaluate every queued pair (dpd_maths_core_half_step_raw.hpp:129-216
+            // with the terms grouped as a wr + sgw (scale u - sgw rdotv)), then add
+            // each lane's own pairs to its row sums in acceptance order.
+            auto flush = [&]() {
+                wave_sync();
+                for (uint32_t i = (uint32_t)lane; i < queued; i += kForceBeads) {
+                    const uint32_t e = qslot[i];
+                    const uint32_t hl = qhome[i];
+                    const float4 op = pos[e & kSlotMask], ov = vel[e & kSlotMask];
+                    const float4 p = s_hp[hl], v = s_hv[hl];
+                    const float ox_ = op.x + shift_of((e >> 26) & 3u, Lx);
+                    const float oy_ = op.y + shift_of((e >> 28) & 3u, Ly);
+                    const float oz_ = op.z + shift_of((e >> 30) & 3u, Lz);
+                    const float dx = p.x - ox_, dy = p.y - oy_, dz = p.z - oz_;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float rr = sqrtf(r2);
+                    const float inv_r = 1.0f / rr;
+                    const float wr = 1.0f - rr;
+                    const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+                    const float dvx = v.x - ov.x, dvy = v.y - ov.y, dvz = v.z - ov.z;
+                    const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+                    const uint32_t ph = fbits(p.w), oh = fbits(op.w);
+                    const uint32_t ti = (ph >> 28) * kMaxTypes + (oh >> 28);
+                    const float sgw = s_sg[ti] * wr;
+                    const float u = rng_uniform(rng_sym(th_lo, th_hi, ph, oh));
+                    const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                    qres[i] = make_float4(ex, ey, ez, F);
+                }
+                wave_sync();
                 uint32_t c = 0;
-                for (; c + 2 <= cnt[r]; c += 2) {
-                    const uint32_t q0 = s_list[r][c][tid], q1 = s_list[r][c + 1][tid];
-                    const uint32_t m0 = slot_of(r, q0), m1 = slot_of(r, q1);
-                    const float4 op0 = pos[m0], ov0 = vel[m0];
-                    const float4 op1 = pos[m1], ov1 = vel[m1];
-                    pair(code_of(r, q0), op0, ov0, px[r], py[r], pz[r]);
-                    pair(code_of(r, q1), op1, ov1, px[r], py[r], pz[r]);
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    for (uint32_t k = 0; k < cnt[r]; k++, c++) {
+                        const float4 q = qres[s_list[c][tid]];
+                        px[r] = fmaf(q.x, q.w, px[r]);
+                        py[r] = fmaf(q.y, q.w, py[r]);
+                        pz[r] = fmaf(q.z, q.w, pz[r]);
+                    }
+                    cnt[r] = 0;
                 }
-                if (c < cnt[r]) {
-                    const uint32_t q = s_list[r][c][tid], m = slot_of(r, q);
-                    pair(code_of(r, q), pos[m], vel[m], px[r], py[r], pz[r]);
-                }
-                cnt[r] = 0;
+                queued = 0;
+                wave_sync();
             };
-            const uint32_t len0 = na[0] + nb[0], len1 = na[1] + nb[1], len2 = na[2] + nb[2];
-            uint32_t maxlen = max(len0, max(len1, len2));
-            if (maxlen > 0xffffu) {           // stream positions are kept in 16 bits
-                atomicOr(S.flags, PDPD_FLAG_DENSE_ROW);
-                maxlen = 0;
-            }
-            for (uint32_t k = 0; k < maxlen; k += kScanWidth) {
+
+            // Rows in turn, kScanWidth candidates per lane at a time. The queue is
+            // flushed when the next batch would overflow it or a lane's list, and
+            // at the end.
 #pragma unroll
-                for (int r = 0; r < 3; r++)
-                    if (cnt[r] > kRowList - kScanWidth) flush(r);
-                uint32_t slot[3][kScanWidth];
-                float3 p[3][kScanWidth];
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    const uint32_t len = na[r] + nb[r];
+            for (int r = 0; r < 3; r++) {
+                const uint32_t len = na[r] + nb[r];
+                const uint32_t wlen = wave_max(len);
+                const float sy = shift_of((cxy[r] >> 28) & 3u, Ly);
+                for (uint32_t k = 0; k < wlen; k += kScanWidth) {
+                    uint32_t slot[kScanWidth];
+                    float3 pc[kScanWidth];
 #pragma unroll
                     for (int j = 0; j < kScanWidth; j++) {
                         const uint32_t q = k + j;
-                        // beyond the row's stream: the home slot, a valid address that is never accepted
-                        slot[r][j] = q >= len ? s : (q < na[r] ? ra[r] + q : rb[r] + (q - na[r]));
-                        p[r][j] = load_xyz(pos, slot[r][j]);
+                        // beyond the row's stream: the home slot, never accepted (r2 = 0)
+                        slot[j] = q >= len ? sl : (q < na[r] ? ra[r] + q : rb[r] + (q - na[r]));
+                        pc[j] = load_xyz(pos, slot[j]);
                     }
-                }
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
+                    uint32_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < kScanWidth; j++) {
-                        const uint32_t q = k + j;
-                        const bool inA = q < na[r];
-                        const float ozs = inA ? szA : szB;
-                        const float sy = shift_of((cxy[r] >> 28) & 3u, Ly);
-                        const float dx = hp.x - (p[r][j].x + sx), dy = hp.y - (p[r][j].y + sy),
-                                    dz = hp.z - (p[r][j].z + ozs);
+                        const float ozs = k + j < na[r] ? szA : szB;
+                        const float dx = hp.x - (pc[j].x + sx), dy = hp.y - (pc[j].y + sy),
+                                    dz = hp.z - (pc[j].z + ozs);
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        if (r2 < 1.0f && r2 >= kMinR2 && slot[r][j] != s) s_list[r][cnt[r]++][tid] = (uint16_t)q;
+                        if (r2 < 1.0f && r2 >= kMinR2) acc |= 1u << j;
                     }
+                    unsigned long long bal[kScanWidth];
+                    uint32_t total = 0;
+#pragma unroll
+                    for (int j = 0; j < kScanWidth; j++) {
+                        bal[j] = __ballot((acc >> j) & 1u);
+                        total += (uint32_t)__popcll(bal[j]);
+                    }
+                    if (total == 0) continue;
+                    const uint32_t mine = (uint32_t)__popc(acc);
+                    if (queued + total > (uint32_t)kQueue ||
+                        __ballot(cnt[0] + cnt[1] + cnt[2] + mine > (uint32_t)kLaneList))
+                        flush();
+                    uint32_t at = queued, c = cnt[0] + cnt[1] + cnt[2];
+#pragma unroll
+                    for (int j = 0; j < kScanWidth; j++) {
+                        if ((acc >> j) & 1u) {
+                            const uint32_t qp = at + (uint32_t)__popcll(bal[j] & below);
+                            qslot[qp] = slot[j] | cxy[r] | (k + j < na[r] ? zcA : zcB);
+                            qhome[qp] = (uint8_t)lane;
+                            s_list[c++][tid] = (uint8_t)qp;
+                        }
+                        at += (uint32_t)__popcll(bal[j]);
+                    }
+                    cnt[r] += mine;
+                    queued = at;
                 }
             }
-#pragma unroll
-            for (int r = 0; r < 3; r++) flush(r);
+            if (queued) flush();
             fx = (px[0] + px[1]) + px[2];
             fy = (py[0] + py[1]) + py[2];
             fz = (pz[0] + pz[1]) + pz[2];
         }
-        if (!kBondWave && plane == 1) bonded_terms(S, W, pos, vel, s, moving, lane, s_bond, s_nterms);
+        if (!kBondWave && plane == 1) bonded_terms(S, vel, s, moving, lane, s_bond, s_nterms);
         // combine the three planes in order: (p0 + p1) + p2
         if (plane > 0 && plane < kForcePlanes) {
             s_part[plane - 1][0][lane] = fx;
@@ -826,14 +887,15 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
     DeviceState Sp = L.S;
     Sp.ctr = L.S.ctr + L.cur * L.n_worlds;       // counters read by that step
     const int nxt = L.cur ^ 1;
+    if (L.n_terms) k_terms<<<ceil_div(L.n_terms, kBeadThreads), kBeadThreads, 0, st>>>(Sp, nxt, L.n_terms);
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     if (fg) {
         constexpr uint32_t t3 = kForceBeads * kForcePlanes;
         switch (L.force_variant) {
-        case 1: k_force<3, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 1: k_force<4, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
         case 2: k_force<2, 5, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        case 3: k_force<1, 6, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        default: k_force<2, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 3: k_force<3, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        default: k_force<3, 5, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
         }
     }
     return hipGetLastError();

@@ -15,13 +15,14 @@ struct W {
     pdpd_world_desc d{};
 };
 
-static void make(W &w, int Lx, int Ly, int Lz, double bonded, bool wall, double theta0, uint64_t seed)
+static void make(W &w, int Lx, int Ly, int Lz, double bonded, bool wall, double theta0, uint64_t seed,
+                 int density = 3)
 {
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<double> U(0, 1);
     std::normal_distribution<double> N(0, 1);
     w.box[0] = Lx; w.box[1] = Ly; w.box[2] = Lz;
-    const int n = 3 * Lx * Ly * Lz;
+    const int n = density * Lx * Ly * Lz;
     const int clen = 6;
     const int nch = (int)(bonded * n / clen);
     const int nt = wall ? 4 : 3;
@@ -86,11 +87,12 @@ int main(int argc, char **argv)
         {"wall_v2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=2"},
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
+        {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
     };
     for (auto &c : cases) {
         fprintf(stderr, "case %s\n", c.name);
         W w;
-        make(w, c.L[0], c.L[1], c.L[2], c.bonded, c.wall, c.th, 7);
+        make(w, c.L[0], c.L[1], c.L[2], c.bonded, c.wall, c.th, 7, strcmp(c.name, "dense") ? 3 : 24);
         const char *why = pdpd_can_support(&w.d);
         if (why && *why) { fprintf(stderr, "unsupported: %s\n", why); return 1; }
         pdpd_ctx *ctx;

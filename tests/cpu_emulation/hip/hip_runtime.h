@@ -34,6 +34,7 @@ inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long
 inline int __ffsll(long long x) { return x ? __builtin_ctzll((unsigned long long)x) + 1 : 0; }
 inline int __ffsll(unsigned long long x) { return x ? __builtin_ctzll(x) + 1 : 0; }
 inline unsigned __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned __popc(unsigned x) { return __builtin_popcount(x); }
 template <class A, class B> inline auto min(A a, B b) { return a < b ? a : b; }
 template <class A, class B> inline auto max(A a, B b) { return a > b ? a : b; }
 
@@ -172,6 +173,7 @@ inline uint32_t __shfl(uint32_t v, int src, int = 64) { return (uint32_t)emu::wa
 inline uint32_t __shfl_xor(uint32_t v, int m, int = 64) { return (uint32_t)emu::wave_op(3, v, m); }
 inline void __builtin_amdgcn_s_sleep(int) { emu::yield_(); }
 inline void __builtin_amdgcn_wave_barrier() { (void)emu::wave_op(4, 0, 0); }
+inline void __builtin_amdgcn_fence(int, const char *) {}   // fibers run one at a time
 template <class T, class U> inline T atomicAdd(T *p, U v) { T o = *p; *p = o + (T)v; return o; }
 template <class T, class U> inline T atomicOr(T *p, U v) { T o = *p; *p = o | (T)v; return o; }
 template <class T> inline T __hip_atomic_load(T *p, int, int) { return *p; }

@@ -8,7 +8,9 @@ live work-items, and a wave intrinsic reached by only part of a wave aborts.
     persistent grid, every pair-force variant) through the C ABI: no
     out-of-bounds access, no undefined behaviour, no divergent wave intrinsic;
   * the plain build's first step equals the FP32 oracle bit for bit (the same
-    source as the gfx950 build, FP32 with -ffp-contract=off).
+    source as the gfx950 build, FP32 with -ffp-contract=off), including a dense
+    world whose accepted pairs overflow the pair kernel's per-bead lists and
+    per-wave queues many times.
 This checks the kernel source on the CPU; GPU parity is tests/test_gpu_*.py."""
 import os
 import subprocess
@@ -51,8 +53,9 @@ worlds = {
     "wall_bent6": W.add_wall(W.make_world((6, 6, 6), bonded_fraction=0.2, seed=15, theta0=1.0,
                                           a_matrix=W.random_a_matrix(rngA))),
     "lattice4": W.make_lattice_world(),
+    "dense3": W.make_world((3, 4, 3), density=24.0, seed=16),    # pair lists and queues overflow
 }
-for (name, w), var in zip(worlds.items(), (0, 1, 2, 3)):
+for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2)):
     ctx = capi.Context(0)
     ctx.set_config("force_variant", var)
     ctx.set_config("scan_lookback_tiles", (var & 1) * 64)     # both cell-scan forms
@@ -74,4 +77,4 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 4, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 5, r.stdout
