@@ -248,6 +248,10 @@ int relayout(pdpd_ctx *ctx)
         HIP_TRY(dev_alloc(nvel[b], nb));
         HIP_TRY(dev_alloc(nfrc[b], nb));
     }
+    // slot_of_bead describes the buffer each step starts from: moved along with
+    // resident worlds, the identity for uploaded ones
+    DevBuf<uint32_t> nsob;
+    HIP_TRY(dev_alloc(nsob, nb));
     for (size_t i = 0; i < order.size(); i++) {
         WorldHost &w = ctx->worlds.at(order[i]);
         const uint32_t off = new_bead_off[i];
@@ -261,7 +265,9 @@ int relayout(pdpd_ctx *ctx)
             HIP_TRY(hipMemcpyAsync(nfrc[0].p + off, ctx->frc[c].p + w.bead_off, w.n * sizeof(float4),
                                    hipMemcpyDeviceToDevice, ctx->stream));
             HIP_TRY(launch_rebase_index(nvel[0].p + off, w.n, off - w.bead_off, ctx->stream));
+            HIP_TRY(launch_rebase_slots(ctx->slot_of_bead.p, nsob.p, w.n, w.bead_off, off, ctx->stream));
         } else {
+            HIP_TRY(launch_rebase_slots(nullptr, nsob.p, w.n, 0, off, ctx->stream));
             for (uint32_t k = 0; k < w.n; k++) w.up_vel[k].w = u2f(off + k);
             HIP_TRY(hipMemcpyAsync(npos[0].p + off, w.up_pos.data(), w.n * sizeof(float4), hipMemcpyHostToDevice,
                                    ctx->stream));
@@ -276,6 +282,8 @@ int relayout(pdpd_ctx *ctx)
         ctx->pos[b].release(); ctx->vel[b].release(); ctx->frc[b].release();
         ctx->pos[b] = npos[b]; ctx->vel[b] = nvel[b]; ctx->frc[b] = nfrc[b];
     }
+    ctx->slot_of_bead.release();
+    ctx->slot_of_bead = nsob;
     ctx->cur = 0;
     std::vector<const WorldHost *> ws;
     for (size_t i = 0; i < order.size(); i++) {
@@ -299,7 +307,6 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_alloc(ctx->rank, nb));
     HIP_TRY(dev_alloc(ctx->perm, nb));
     HIP_TRY(dev_alloc(ctx->sorted_idx, nb));
-    HIP_TRY(dev_alloc(ctx->slot_of_bead, nb));
     HIP_TRY(dev_alloc(ctx->cell_count, nc));
     HIP_TRY(dev_alloc(ctx->cell_start, nc + 1));
     HIP_TRY(dev_alloc(ctx->scan_tiles, scan_tiles((uint32_t)nc)));
@@ -961,19 +968,24 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const size_t nb = ctx->n_beads;
     DevBuf<float4> sp, sv, sf;
+    DevBuf<uint32_t> ss;
     HIP_TRY(dev_alloc(sp, nb));
     HIP_TRY(dev_alloc(sv, nb));
     HIP_TRY(dev_alloc(sf, nb));
+    HIP_TRY(dev_alloc(ss, nb));
     const int c0 = ctx->cur;
     // save and restore on the engine's stream: the stream is non-blocking, so a
     // plain hipMemcpy would not be ordered with the trial steps
     HIP_TRY(hipMemcpyAsync(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(sv.p, ctx->vel[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(sf.p, ctx->frc[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ss.p, ctx->slot_of_bead.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     auto restore = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->slot_of_bead.p, ss.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               ctx->stream));
         ctx->cur = c0;
         return PDPD_OK;
     };

@@ -52,7 +52,7 @@ struct BeadBlock {
 //   kind 0 (Hookean): a = head, b = tail, p0 = r0
 //   kind 1 (angle):   a = head, b = middle, c = tail, p0 = cos(theta0), p1 = sin(theta0)
 // kind holds the term kind in bits 0-7 and the world (index into the world
-// table) in bits 8-31. Bonded terms are evaluated in FP64 (k_terms), once per
+// table) in bits 8-31. Bonded terms are evaluated in FP64 (during k_scatter), once per
 // term per step, into term_frc.
 struct TermRec { uint32_t a, b, c, kind; double kappa, p0, p1; };
 constexpr uint32_t kTermBond = 0, kTermAngle = 1;
@@ -85,7 +85,7 @@ struct DeviceState {
     uint32_t *cell_start;  // n_cells_total + 1
     unsigned long long *scan_tiles;   // look-back state per scan tile
     uint32_t *scan_ticket;
-    uint32_t *slot_of_bead;
+    uint32_t *slot_of_bead;    // bonded (and stationary) beads: slot in the buffer a step starts from
     uint32_t *flags;
     WorldDev *worlds;
     WorldCtr *ctr;
@@ -94,7 +94,7 @@ struct DeviceState {
     uint32_t *term_start;      // n_beads + 1, by global bead index
     uint32_t *bead_terms;
     TermRec *terms;
-    double *term_frc;          // kTermFrcStride per term, written by k_terms each step
+    double *term_frc;          // kTermFrcStride per term, written during k_scatter each step
     StationaryBead *stationary_beads;
 };
 

@@ -32,11 +32,14 @@ struct KernelEvents {
 
 uint32_t scan_tiles(uint32_t n_cells);
 hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
-// The force launches of the step L describes (bonded terms, then pair forces;
-// they read and write buffer L.cur ^ 1).
+// The pair-force launch of the step L describes (reads and writes buffer
+// L.cur ^ 1; the bonded terms were evaluated during the step's scatter).
 hipError_t launch_force(const StepLaunch &L, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
+// slot_of_bead of a world moved from bead offset old_off to new_off (old = null: identity).
+hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
+                               hipStream_t st);
 hipError_t launch_debug_rng(uint32_t n, const uint64_t *th, const uint32_t *a, const uint32_t *b, uint32_t *u32,
                             float *f32, hipStream_t st);
 

@@ -6,11 +6,11 @@
 //                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does); advances
 //                 the per-world step counters
 //   k_scan        single-pass exclusive scan of the cell counts (look-back)
-//   k_scatter     sorted position of every bead's slot
-//   k_reorder     stable gather into the other buffer, bead-index order in a cell
-//   k_terms       one thread per Hookean bond / angle: its forces in FP64
-//                 (terms of naive_dpd_engine_half_merge.hpp:638-690 /
+//   k_scatter     sorted position of every bead's slot; extra workgroups
+//                 evaluate the bonded terms (one thread per Hookean bond / angle,
+//                 FP64; terms of naive_dpd_engine_half_merge.hpp:638-690 /
 //                 dpd_maths_core_half_step.hpp:178-231), once per term
+//   k_reorder     stable gather into the other buffer, bead-index order in a cell
 //   k_force       per workgroup 64 consecutive beads: three waves for the pair
 //                 forces, one per x-plane of the 27-cell stencil, neighbours read
 //                 straight from the cell-sorted arrays; register accumulation, no
@@ -373,16 +373,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
 
 // ------------------------------------------------------- sort scatter ----
 
-__global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur, uint32_t n)
-{
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t r = S.rank[s];
-    const uint32_t j = S.cell_start[S.key[s]] + (r & ~kInactive);
-    S.perm[j] = s | (r & kInactive);
-    S.sorted_idx[j] = fbits(S.vel[cur][s].w);
-}
-
 // Stable order inside a cell: beads ranked by global bead index, so the order
 // (and every force sum) is independent of arrival order and of the batch.
 // slot_of_bead is read only for bonded beads (the bonded terms) and stationary
@@ -425,14 +415,15 @@ __device__ __forceinline__ double min_image(double d, double len)
 // dpd_maths_core_half_step.hpp:178-231 with the prefactor floor of
 // naive_dpd_engine.hpp:681-682, the argument clamped at 0 (stored: head, then
 // tail; the middle gets -(head + tail)). Terms of frozen worlds are skipped.
-__global__ __launch_bounds__(kBeadThreads) void k_terms(DeviceState S, int nxt, uint32_t n_terms)
+// Reads the drifted positions in buffer cur through slot_of_bead, which maps
+// each bonded bead to its slot in that buffer (the previous step's sort, or
+// the layout an import or relayout produced).
+__device__ __forceinline__ void eval_term(const DeviceState &S, int cur, uint32_t k)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_terms) return;
     const TermRec R = S.terms[k];
     const uint32_t w = R.kind >> kTermWorldShift;
     if (S.ctr[w].remaining == 0) return;
-    const float4 *__restrict__ pos = S.pos[nxt];
+    const float4 *__restrict__ pos = S.pos[cur];
     const double Lx = S.worlds[w].Lf[0], Ly = S.worlds[w].Lf[1], Lz = S.worlds[w].Lf[2];
     double *__restrict__ out = S.term_frc + (size_t)k * kTermFrcStride;
     const float4 pa = pos[S.slot_of_bead[R.a]], pb = pos[S.slot_of_bead[R.b]];
@@ -474,6 +465,34 @@ __global__ __launch_bounds__(kBeadThreads) void k_terms(DeviceState S, int nxt, 
     out[3] = tx; out[4] = ty; out[5] = tz;
 }
 
+// Sort scatter: the sorted position of every bead's slot (n_scatter_blocks
+// workgroups), then the bonded terms (the remaining workgroups).
+__global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur, uint32_t n,
+                                                          uint32_t n_scatter_blocks, uint32_t n_terms)
+{
+    if (blockIdx.x >= n_scatter_blocks) {
+        const uint32_t k = (blockIdx.x - n_scatter_blocks) * blockDim.x + threadIdx.x;
+        if (k < n_terms) eval_term(S, cur, k);
+        return;
+    }
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t r = S.rank[s];
+    const uint32_t j = S.cell_start[S.key[s]] + (r & ~kInactive);
+    S.perm[j] = s | (r & kInactive);
+    S.sorted_idx[j] = fbits(S.vel[cur][s].w);
+}
+
+// slot_of_bead for a world moved to bead offset new_off: from its mapping at
+// old_off (resident worlds keep their order), or the identity (worlds uploaded
+// in bead order).
+__global__ void k_rebase_slots(const uint32_t *__restrict__ old, uint32_t *__restrict__ nw, uint32_t n,
+                               uint32_t old_off, uint32_t new_off)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) nw[new_off + b] = old ? old[old_off + b] - old_off + new_off : new_off + b;
+}
+
 // --------------------------------------------------------- pair force ----
 
 // Shift codes of a candidate's periodic image, 2 bits per axis: 0 none,
@@ -499,13 +518,20 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Slots are below 2^26, so the byte offset of a float4 fits 32 bits: the loads
+// take a scalar base and a 32-bit vector offset.
+__device__ __forceinline__ const float4 &at_slot(const float4 *__restrict__ p, uint32_t i)
+{
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(p) + (i << 4));
+}
+
 __device__ __forceinline__ float3 load_xyz(const float4 *__restrict__ p, uint32_t i)
 {
-    const float *q = reinterpret_cast<const float *>(p + i);
+    const float *q = reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + (i << 4));
     return make_float3(q[0], q[1], q[2]);
 }
 
-// The bonded terms of the bead in slot s (computed by k_terms), summed in FP64
+// The bonded terms of the bead in slot s (evaluated during k_scatter), summed in FP64
 // in the per-polymer order into s_bond / s_nterms (lane = the bead's lane in
 // its workgroup).
 __device__ __forceinline__ void bonded_terms(const DeviceState &S, const float4 *__restrict__ vel, uint32_t s,
@@ -678,6 +704,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
             float4 *__restrict__ qres = s_qres[plane];
             float px[3] = {0.0f, 0.0f, 0.0f}, py[3] = {0.0f, 0.0f, 0.0f}, pz[3] = {0.0f, 0.0f, 0.0f};
             uint32_t cnt[3] = {0u, 0u, 0u};     // this lane's queued pairs per row
+            uint32_t listed = 0;                // and in all (its list length)
             uint32_t queued = 0;                // queue fill, the same in every lane
 
             // Evaluate every queued pair (dpd_maths_core_half_step_raw.hpp:129-216
@@ -688,7 +715,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 for (uint32_t i = (uint32_t)lane; i < queued; i += kForceBeads) {
                     const uint32_t e = qslot[i];
                     const uint32_t hl = qhome[i];
-                    const float4 op = pos[e & kSlotMask], ov = vel[e & kSlotMask];
+                    const float4 op = at_slot(pos, e & kSlotMask), ov = at_slot(vel, e & kSlotMask);
                     const float4 p = s_hp[hl], v = s_hv[hl];
                     const float ox_ = op.x + shift_of((e >> 26) & 3u, Lx);
                     const float oy_ = op.y + shift_of((e >> 28) & 3u, Ly);
@@ -721,6 +748,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                     cnt[r] = 0;
                 }
                 queued = 0;
+                listed = 0;
                 wave_sync();
             };
 
@@ -731,6 +759,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
             for (int r = 0; r < 3; r++) {
                 const uint32_t len = na[r] + nb[r];
                 const uint32_t wlen = wave_max(len);
+                const uint32_t rbm = rb[r] - na[r];              // slot of stream position q >= na: rbm + q
                 const float sy = shift_of((cxy[r] >> 28) & 3u, Ly);
                 for (uint32_t k = 0; k < wlen; k += kScanWidth) {
                     uint32_t slot[kScanWidth];
@@ -739,42 +768,40 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                     for (int j = 0; j < kScanWidth; j++) {
                         const uint32_t q = k + j;
                         // beyond the row's stream: the home slot, never accepted (r2 = 0)
-                        slot[j] = q >= len ? sl : (q < na[r] ? ra[r] + q : rb[r] + (q - na[r]));
+                        const uint32_t inrow = q < na[r] ? ra[r] + q : rbm + q;
+                        slot[j] = q < len ? inrow : sl;
                         pc[j] = load_xyz(pos, slot[j]);
                     }
-                    uint32_t acc = 0;
+                    bool ok[kScanWidth];
+                    unsigned long long bal[kScanWidth];
+                    uint32_t total = 0;
 #pragma unroll
                     for (int j = 0; j < kScanWidth; j++) {
                         const float ozs = k + j < na[r] ? szA : szB;
                         const float dx = hp.x - (pc[j].x + sx), dy = hp.y - (pc[j].y + sy),
                                     dz = hp.z - (pc[j].z + ozs);
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        if (r2 < 1.0f && r2 >= kMinR2) acc |= 1u << j;
-                    }
-                    unsigned long long bal[kScanWidth];
-                    uint32_t total = 0;
-#pragma unroll
-                    for (int j = 0; j < kScanWidth; j++) {
-                        bal[j] = __ballot((acc >> j) & 1u);
+                        ok[j] = r2 < 1.0f && r2 >= kMinR2;
+                        bal[j] = __builtin_amdgcn_ballot_w64(ok[j]);
                         total += (uint32_t)__popcll(bal[j]);
                     }
                     if (total == 0) continue;
-                    const uint32_t mine = (uint32_t)__popc(acc);
+                    // room for this batch: a lane may list up to kScanWidth more pairs
                     if (queued + total > (uint32_t)kQueue ||
-                        __ballot(cnt[0] + cnt[1] + cnt[2] + mine > (uint32_t)kLaneList))
+                        __builtin_amdgcn_ballot_w64(listed + kScanWidth > (uint32_t)kLaneList))
                         flush();
-                    uint32_t at = queued, c = cnt[0] + cnt[1] + cnt[2];
+                    uint32_t at = queued;
 #pragma unroll
                     for (int j = 0; j < kScanWidth; j++) {
-                        if ((acc >> j) & 1u) {
+                        if (ok[j]) {
                             const uint32_t qp = at + (uint32_t)__popcll(bal[j] & below);
                             qslot[qp] = slot[j] | cxy[r] | (k + j < na[r] ? zcA : zcB);
                             qhome[qp] = (uint8_t)lane;
-                            s_list[c++][tid] = (uint8_t)qp;
+                            s_list[listed++][tid] = (uint8_t)qp;
+                            cnt[r]++;
                         }
                         at += (uint32_t)__popcll(bal[j]);
                     }
-                    cnt[r] += mine;
                     queued = at;
                 }
             }
@@ -869,7 +896,7 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     }
     rec(2);
     const uint32_t gb = ceil_div(L.n_beads, kBeadThreads);
-    k_scatter<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads);
+    k_scatter<<<gb + ceil_div(L.n_terms, kBeadThreads), kBeadThreads, 0, st>>>(Sp, cur, L.n_beads, gb, L.n_terms);
     rec(3);
     k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads, L.n_stationary > 0 ? 1 : 0);
     rec(4);
@@ -887,7 +914,6 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
     DeviceState Sp = L.S;
     Sp.ctr = L.S.ctr + L.cur * L.n_worlds;       // counters read by that step
     const int nxt = L.cur ^ 1;
-    if (L.n_terms) k_terms<<<ceil_div(L.n_terms, kBeadThreads), kBeadThreads, 0, st>>>(Sp, nxt, L.n_terms);
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     if (fg) {
         constexpr uint32_t t3 = kForceBeads * kForcePlanes;
@@ -905,6 +931,14 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     k_fill_u32<<<ceil_div(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
+                               hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    k_rebase_slots<<<ceil_div(n, 256), 256, 0, st>>>(old, nw, n, old_off, new_off);
     return hipGetLastError();
 }
 

@@ -114,6 +114,14 @@ int main(int argc, char **argv)
             CHECK(pdpd_world_set_steps(ctx, s3, 1));
             CHECK(pdpd_run(ctx, c.steps));
             CHECK(pdpd_sync(ctx));
+            W w4;                                 // a world joins, one leaves: resident worlds move
+            make(w4, 5, 4, 6, 0.2, true, 0.5, 13);
+            int s4;
+            CHECK(pdpd_world_add(ctx, &w4.d, &s4));
+            CHECK(pdpd_run(ctx, 1));
+            CHECK(pdpd_world_remove(ctx, s2));
+            CHECK(pdpd_run(ctx, 1));
+            CHECK(pdpd_sync(ctx));
         } else {
             CHECK(pdpd_run(ctx, c.steps));
             CHECK(pdpd_sync(ctx));

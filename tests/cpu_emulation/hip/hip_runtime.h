@@ -168,6 +168,7 @@ inline void launch(uint32_t grid, uint32_t block, std::function<void()> body)
 #define gridDim (emu::st().gdim)
 inline void __syncthreads() { emu::me().wait = 1; emu::yield_(); }
 inline unsigned long long __ballot(int p) { return emu::wave_op(0, p ? 1 : 0, 0); }
+inline unsigned long long __builtin_amdgcn_ballot_w64(bool p) { return __ballot(p); }
 inline uint32_t __shfl_up(uint32_t v, unsigned d, int = 64) { return (uint32_t)emu::wave_op(1, v, (int)d); }
 inline uint32_t __shfl(uint32_t v, int src, int = 64) { return (uint32_t)emu::wave_op(2, v, src); }
 inline uint32_t __shfl_xor(uint32_t v, int m, int = 64) { return (uint32_t)emu::wave_op(3, v, m); }

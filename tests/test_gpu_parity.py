@@ -176,6 +176,59 @@ def test_step_budgets_freeze_worlds(gpu_lib):
         c2.close()
 
 
+def _same_f32(a, b):
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    assert a[3] == b[3]
+
+
+def test_worlds_added_and_removed_midrun_bitwise(gpu_lib):
+    """A resident bonded world keeps its trajectory bit for bit while other
+    worlds join and leave (each change moves it to a new offset in the device
+    layout, with its bonded beads' slots)."""
+    rng = np.random.default_rng(8)
+    wa = W.add_wall(W.make_world((7, 6, 8), bonded_fraction=0.2, seed=51, theta0=0.6, a_matrix=W.random_a_matrix(rng)))
+    wb = W.make_world((5, 5, 6), bonded_fraction=0.15, seed=52, a_matrix=W.random_a_matrix(rng))
+    wc = W.make_world((6, 4, 5), bonded_fraction=0.1, seed=53)
+    _say("world joins and leaves around a resident world, 4 x 5 steps")
+    ctx = capi.Context(0)
+    sb = ctx.add(wb)             # wa lands after wb; when wb leaves, wa moves down
+    sa = ctx.add(wa)
+    ctx.run(5)
+    sc = ctx.add(wc)
+    ctx.run(5)
+    ctx.remove(sb)
+    ctx.run(5)
+    ctx.remove(sc)
+    ctx.run(5)
+    ctx.sync()
+    got = ctx.download_f32(sa)
+    ctx.close()
+    c2, s2 = run_gpu(wa, 20)
+    _same_f32(c2.download_f32(s2), got)
+    c2.close()
+
+
+def test_autotune_leaves_state_unchanged(gpu_lib):
+    """Autotuning runs trial steps on the attached worlds and restores them: the
+    trajectory afterwards is the untuned one, bit for bit."""
+    w = W.make_world((10, 9, 8), bonded_fraction=0.2, seed=61, theta0=0.4,
+                     a_matrix=W.random_a_matrix(np.random.default_rng(6)))
+    _say("autotune between steps 6 and 12")
+    ctx = capi.Context(0)
+    s = ctx.add(w)
+    ctx.run(6)
+    ctx.sync()
+    ctx.autotune(4)
+    ctx.run(6)
+    ctx.sync()
+    got = ctx.download_f32(s)
+    ctx.close()
+    c2, s2 = run_gpu(w, 12)
+    _same_f32(c2.download_f32(s2), got)
+    c2.close()
+
+
 @pytest.mark.parametrize("cfg", [dict(graph_steps=0), dict(graph_steps=8), dict(force_grid=64),
                                  dict(force_grid=1024, graph_steps=6),
                                  dict(scan_lookback_tiles=0), dict(scan_lookback_tiles=0, graph_steps=4),
