@@ -8,6 +8,15 @@
 //                       launch set per step (the batched pattern; no counterpart
 //                       in the reference)
 //
+// With a force logger installed (ForceLogging::set_logger), HipDPDEngine runs
+// one step at a time and logs what the reference's AVX2 half-merge engine logs
+// (dpd_engine_avx2_half_merge_tbb.hpp:675-760,1244-1259;
+// dpd_maths_core_simd.hpp:232-275): per bead b_hash, x, v, f before the step
+// and v_next, f_next after it; per evaluated pair dx, dr, dv, the dissipative
+// strength, 1/sqrt(dt) scale, gammap, the random draw, the conservative,
+// dissipative and random terms, the random scale and the pair force; per
+// Hookean bond its dx, dr and force, both ways.
+//
 // The header compiles against this repository's restatement of the engine
 // interface (pdpd/dpd_engine.hpp) or, with PDPD_HOST_REFERENCE defined and the
 // reference's include/ on the path, against poets-dpd's own
@@ -16,8 +25,10 @@
 
 #if defined(PDPD_HOST_REFERENCE)
 #include "dpd/core/dpd_engine.hpp"
+#include "dpd/core/logging.hpp"
 #else
 #include "pdpd/dpd_engine.hpp"
+#include "pdpd/force_logging.hpp"
 #endif
 #include "pdpd_hip.h"
 
@@ -265,6 +276,10 @@ public:
     {
         require_attached();
         if (!m_snap.same(*m_state)) reimport();
+        if (ForceLogging::logger()) {
+            for (unsigned i = 0; i < nSteps; i++) logged_step();
+            return;
+        }
         pdpd_host::check(pdpd_run(ctx(), nSteps), "pdpd_run");
         export_state();
     }
@@ -278,6 +293,16 @@ public:
         if (interval_count == 0) return 0;
         if (!m_snap.same(*m_state)) reimport();
         unsigned done = 0;
+        if (ForceLogging::logger()) {     // logged: one step at a time, no interval in flight
+            for (;;) {
+                for (unsigned i = 0; i < interval_size; i++) logged_step();
+                done += interval_size;
+                const bool cont = interval_callback();
+                if (interval_count > 0) --interval_count;
+                if (!cont || interval_count == 0) return done;
+                if (!m_snap.same(*m_state)) reimport();
+            }
+        }
         pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
         for (;;) {
             export_state();
@@ -333,6 +358,68 @@ protected:
     {
         pdpd_host::download_into(ctx(), m_slot, *m_state);
         m_snap.take(*m_state);
+    }
+
+    // One step with the force logger: the state before, the step's pair and
+    // bond evaluations (pdpd_trace_forces), the state after.
+    void logged_step()
+    {
+        ForceLogging *log = ForceLogging::logger();
+        log->SetTime(m_state->t);
+        for (const auto &b : m_state->beads) {
+            const BeadHash h = b.get_hash_code();
+            const double hv = (double)h.hash;
+            const double x[3] = {b.x[0], b.x[1], b.x[2]}, v[3] = {b.v[0], b.v[1], b.v[2]},
+                         f[3] = {b.f[0], b.f[1], b.f[2]};
+            log->LogBeadProperty(h, "b_hash", 1, &hv);
+            log->LogBeadProperty(h, "x", 3, x);
+            log->LogBeadProperty(h, "v", 3, v);
+            log->LogBeadProperty(h, "f", 3, f);
+        }
+        pdpd_host::check(pdpd_run(ctx(), 1), "pdpd_run");
+        uint32_t np = 0, nb = 0;
+        pdpd_host::check(pdpd_trace_forces(ctx(), m_slot, nullptr, 0, &np, nullptr, 0, &nb), "pdpd_trace_forces");
+        std::vector<pdpd_pair_record> pairs(np);
+        std::vector<pdpd_bond_record> bonds(nb);
+        pdpd_host::check(pdpd_trace_forces(ctx(), m_slot, pairs.data(), np, &np, bonds.data(), nb, &nb),
+                         "pdpd_trace_forces");
+        for (const pdpd_pair_record &r : pairs) {
+            const BeadHash h{r.home}, o{r.other};
+            const double dx[3] = {r.dx[0], r.dx[1], r.dx[2]}, dv[3] = {r.dv[0], r.dv[1], r.dv[2]},
+                         ff[3] = {r.f[0], r.f[1], r.f[2]};
+            const double one[8] = {r.dr, r.diss_strength, r.inv_root_dt, r.gammap, r.rng, r.con, r.diss, r.rng_scale};
+            const double rnd = r.rand;
+            log->LogBeadPairProperty(h, o, "dx", 3, dx);
+            log->LogBeadPairProperty(h, o, "dr", 1, &one[0]);
+            log->LogBeadPairProperty(h, o, "dv", 3, dv);
+            log->LogBeadPairProperty(h, o, "dpd-diss-strength", 1, &one[1]);
+            log->LogBeadPairProperty(h, o, "dpd-invrootdt", 1, &one[2]);
+            log->LogBeadPairProperty(h, o, "dpd-gammap", 1, &one[3]);
+            log->LogBeadPairProperty(h, o, "dpd-rng", 1, &one[4]);
+            log->LogBeadPairProperty(h, o, "dpd-con", 1, &one[5]);
+            log->LogBeadPairProperty(h, o, "dpd-diss", 1, &one[6]);
+            log->LogBeadPairProperty(h, o, "dpd-rng-scale", 1, &one[7]);
+            log->LogBeadPairProperty(h, o, "dpd-rand", 1, &rnd);
+            log->LogBeadPairProperty(h, o, "f_next_dpd", 3, ff);
+        }
+        for (const pdpd_bond_record &r : bonds) {
+            const BeadHash h{r.head}, t{r.tail};
+            double dx[3] = {r.dx[0], r.dx[1], r.dx[2]}, ff[3] = {r.f_head[0], r.f_head[1], r.f_head[2]};
+            log->LogBeadPairProperty(h, t, "f_next_hookean_dx", 3, dx);
+            log->LogBeadPairProperty(h, t, "f_next_hookean_dr", 1, &r.dr);
+            log->LogBeadPairProperty(h, t, "f_next_hookean", 3, ff);
+            for (int d = 0; d < 3; d++) { dx[d] = -dx[d]; ff[d] = -ff[d]; }
+            log->LogBeadPairProperty(t, h, "f_next_hookean", 3, ff);
+            log->LogBeadPairProperty(t, h, "f_next_hookean_dx", 3, dx);
+            log->LogBeadPairProperty(t, h, "f_next_hookean_dr", 1, &r.dr);
+        }
+        export_state();
+        for (const auto &b : m_state->beads) {
+            const double v[3] = {b.v[0], b.v[1], b.v[2]}, f[3] = {b.f[0], b.f[1], b.f[2]};
+            log->LogBeadProperty(b.get_hash_code(), "v_next", 3, v);
+            log->LogBeadProperty(b.get_hash_code(), "f_next", 3, f);
+        }
+        log->Flush();
     }
 
     pdpd_host::Context m_ctx;

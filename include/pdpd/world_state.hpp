@@ -107,6 +107,9 @@ struct BeadHash {
         return is_monomer() ? 0 : ((hash >> POLYMER_OFFSET_SHIFT) & POLYMER_OFFSET_MASK);
     }
     uint32_t get_polymer_id() const { return hash & POLYMER_ID_MASK; }
+    // The identity without the type bits (dpd_state.hpp:40,103-104).
+    static const unsigned REDUCED_HASH_MASK = 0xFFFFFFFFu >> BEAD_TYPE_BITS;
+    BeadHash reduced_hash() const { return BeadHash(hash & REDUCED_HASH_MASK); }
     bool operator==(const BeadHash &o) const { return hash == o.hash; }
     bool operator!=(const BeadHash &o) const { return hash != o.hash; }
 };

@@ -181,6 +181,42 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps);
  * Needs at least one completed step. Synchronous. */
 int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms);
 
+/* One evaluated pair of a force trace: the quantities the reference's force
+ * logging records per bead pair (dpd_maths_core_simd.hpp:232-275,
+ * dpd_maths_core_half_step_raw.hpp:190-214), from the home bead's side. */
+typedef struct pdpd_pair_record {
+    uint32_t home, other;        /* bead hashes (BeadHash, with the type bits)           */
+    float dx[3];                 /* x_home - x_other, nearest image                       */
+    float dr;                    /* |dx| (< 1)                                            */
+    float dv[3];                 /* v_home - v_other, half-step velocities                */
+    float diss_strength;         /* gamma_ij                                              */
+    float inv_root_dt;           /* sqrt(24 kT / dt)                                      */
+    float gammap;                /* (sqrt(gamma_ij) (1 - dr))^2                           */
+    float rng;                   /* the pair's uniform draw u in [-0.5, 0.5)             */
+    float con, diss, rng_scale, rand;   /* conservative, dissipative, sqrt(gammap) sqrt(24/dt), random */
+    float f[3];                  /* the pair's force on the home bead, as the step used it */
+} pdpd_pair_record;
+
+/* One Hookean bond of a force trace (dpd_engine_avx2_half_merge_tbb.hpp:1244-1259). */
+typedef struct pdpd_bond_record {
+    uint32_t head, tail;         /* bead hashes                                          */
+    double dx[3];                /* x_tail - x_head, nearest image                        */
+    double dr;
+    double f_head[3];            /* force on the head (the tail gets its negation)        */
+} pdpd_bond_record;
+
+/* Force trace of the step that produced a world's current state: every pair
+ * evaluation of that step (one record per moving home bead and neighbour
+ * within the cutoff, sorted by (home, other)) and every Hookean bond. The step
+ * must have run on the current layout (no world added or removed, no autotune
+ * since), else PDPD_ERR_STATE. *n_pairs / *n_bonds receive the counts; records
+ * are written up to the capacities, and PDPD_ERR_INVALID is returned when a
+ * capacity is too small (pass 0 to query the counts). A debugging aid, not a
+ * fast path: one extra kernel and a download. Pairs with a stationary bead
+ * use its stored velocity, which is 0 after the world's first step. */
+int pdpd_trace_forces(pdpd_ctx *ctx, int slot, pdpd_pair_record *pairs, uint32_t pair_capacity,
+                      uint32_t *n_pairs, pdpd_bond_record *bonds, uint32_t bond_capacity, uint32_t *n_bonds);
+
 /* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
  * the worlds do not advance); the choice stays in the configuration. */

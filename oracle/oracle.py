@@ -266,6 +266,10 @@ def ref_lib(fast=False):
         L.ref_run.restype = C.c_int
         L.ref_run.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.ref_run_logged.restype = C.c_int
+        L.ref_run_logged.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]
+        L.ref_force_log_sample.restype = C.c_int
+        L.ref_force_log_sample.argtypes = [C.c_char_p]
         _ref[path] = L
     return _ref[path]
 
@@ -292,3 +296,22 @@ def ref_run(world, engine, nsteps, fast=False, timing=None):
     if rc:
         raise RuntimeError(L.ref_last_error().decode())
     return x, v, f, t.value
+
+
+def ref_run_logged(world, engine, nsteps, log_path):
+    """Run a reference engine with the reference's FileLogger writing its force
+    log to log_path (the reference's force-logging format)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "poets-dpd_amd"))
+    from pdpd_amd.capi import world_desc
+    d, keep = world_desc(world)
+    L = ref_lib()
+    rc = L.ref_run_logged(C.addressof(d), engine.encode(), int(nsteps), str(log_path).encode())
+    if rc:
+        raise RuntimeError(L.ref_last_error().decode())
+
+
+def ref_force_log_sample(path):
+    """The fixed call sequence of tests/cpp/force_log_sample.inc through the
+    reference's FileLogger."""
+    ref_lib().ref_force_log_sample(str(path).encode())

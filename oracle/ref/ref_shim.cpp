@@ -12,6 +12,8 @@
 #include "dpd/engines/naive/naive_dpd_engine.hpp"
 #include "dpd/engines/naive/naive_dpd_engine_half_merge.hpp"
 #include "dpd/engines/naive/naive_dpd_engine_half_step.hpp"
+#include "dpd/core/logging.hpp"
+#include "dpd/core/logging_impl.hpp"
 
 #include "pdpd_hip.h"
 
@@ -104,6 +106,8 @@ static WorldState make_state(const pdpd_world_desc *d)
 
 static std::string g_err;
 
+#include "../../tests/cpp/force_log_sample.inc"
+
 extern "C" {
 
 const char *ref_last_error(void) { return g_err.c_str(); }
@@ -159,6 +163,26 @@ int ref_run(const pdpd_world_desc *d, const char *engine, int nsteps, double *x,
         g_err = sx;
         return -1;
     }
+}
+
+// The same run with the reference's FileLogger writing every logged force
+// property to log_path (the force-log format of logging_impl.hpp).
+int ref_run_logged(const pdpd_world_desc *d, const char *engine, int nsteps, const char *log_path)
+{
+    FileLogger *log = new FileLogger(log_path);
+    ForceLogging::set_logger(log);
+    const int rc = ref_run(d, engine, nsteps, nullptr, nullptr, nullptr, nullptr, nullptr);
+    ForceLogging::set_logger(nullptr);
+    delete log;
+    return rc;
+}
+
+// tests/cpp/force_log_sample.inc through the reference's FileLogger.
+int ref_force_log_sample(const char *path)
+{
+    FileLogger log(path);
+    force_log_sample(log);
+    return 0;
 }
 
 }  // extern "C"

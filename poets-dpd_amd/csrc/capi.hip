@@ -85,6 +85,8 @@ struct WorldHost {
     std::vector<TermRec> bonds, angles;            // local bead indices, reference order
     std::vector<uint32_t> stationary_beads;        // local indices of beads of stationary types
     uint32_t bead_off = 0, cell_off = 0;          // layout
+    uint32_t term_base = 0;                        // index of the world's first bonded term (bonds, then angles)
+    int64_t t_at_layout = 0;                       // t when the current layout was built
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
 };
 
@@ -129,6 +131,7 @@ struct pdpd_ctx {
     int next_slot = 0;
     bool dirty = true;
     uint64_t layout_version = 0;
+    uint64_t steps_on_layout = 0;          // steps run since the layout (and its cell lists) was built
 
     // device layout
     DevBuf<float4> pos[2], vel[2], frc[2];
@@ -353,6 +356,9 @@ int relayout(pdpd_ctx *ctx)
     std::vector<uint32_t> tstart(nb + 1, 0), terms;
     if (ws.size() >= (1u << (32 - kTermWorldShift))) return fail(PDPD_ERR_INVALID, "too many worlds in one context");
     for (size_t i = 0; i < ws.size(); i++) {
+        WorldHost &wm = ctx->worlds.at(order[i]);
+        wm.term_base = (uint32_t)trec.size();
+        wm.t_at_layout = wm.t;
         const WorldHost &w = *ws[i];
         if (w.bonds.empty() && w.angles.empty()) continue;
         std::vector<std::vector<uint32_t>> per(w.n);
@@ -388,6 +394,7 @@ int relayout(pdpd_ctx *ctx)
     ctx->stats.n_beads = ctx->n_beads;
     ctx->stats.n_force_blocks = ctx->n_fblocks;
     ctx->layout_version++;
+    ctx->steps_on_layout = 0;
     ctx->graph_version = ~0ull;
     ctx->dirty = false;
     return PDPD_OK;
@@ -725,6 +732,7 @@ int pdpd_run(pdpd_ctx *ctx, uint32_t nsteps)
     if (rc) return rc;
     finish_counters(ctx, nsteps);
     ctx->stats.steps += nsteps;
+    ctx->steps_on_layout += nsteps;
     ctx->stats.bead_steps += bs;
     ctx->stats.run_s += now_s() - t0;
     return PDPD_OK;
@@ -920,6 +928,7 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps)
     for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev.ev[k]);
     finish_counters(ctx, nsteps);
     ctx->stats.steps += nsteps;
+    ctx->steps_on_layout += nsteps;
     ctx->stats.bead_steps += bs;
     ctx->stats.k_integrate_ms = acc[0] / nsteps;
     ctx->stats.k_scan_ms = acc[1] / nsteps;
@@ -935,7 +944,7 @@ int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms)
 {
     if (!ctx || !avg_ms) return fail(PDPD_ERR_INVALID, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    if (ctx->dirty || ctx->n_worlds == 0 || ctx->stats.steps == 0)
+    if (ctx->dirty || ctx->n_worlds == 0 || ctx->steps_on_layout == 0)
         return fail(PDPD_ERR_STATE, "time_force needs a completed step on the current layout");
     reps = std::max<uint32_t>(1, reps);
     const StepLaunch L = ctx->step_launch(ctx->cur ^ 1);   // the step that produced the current buffers
@@ -953,6 +962,75 @@ int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms)
     (void)hipEventDestroy(e1);
     *avg_ms = ms / reps;
     return check_flags(ctx);
+}
+
+int pdpd_trace_forces(pdpd_ctx *ctx, int slot, pdpd_pair_record *pairs, uint32_t pair_capacity,
+                      uint32_t *n_pairs, pdpd_bond_record *bonds, uint32_t bond_capacity, uint32_t *n_bonds)
+{
+    WorldHost *w = ctx ? find_world(ctx, slot) : nullptr;
+    if (!w) return fail(PDPD_ERR_INVALID, "unknown slot");
+    if ((pair_capacity && !pairs) || (bond_capacity && !bonds)) return fail(PDPD_ERR_INVALID, "null record buffer");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = pdpd_sync(ctx);
+    if (rc) return rc;
+    if (ctx->dirty || ctx->steps_on_layout == 0 || !w->resident || w->t == w->t_at_layout)
+        return fail(PDPD_ERR_STATE, "force trace needs a step of this world on the current layout");
+    uint32_t world = 0;
+    while (world < ctx->order.size() && ctx->order[world] != slot) world++;
+    // pair records: appended on the device, then sorted by (home, other)
+    DevBuf<pdpd_pair_record> drec;
+    DevBuf<uint32_t> dcount;
+    HIP_TRY(dev_alloc(drec, pair_capacity));
+    HIP_TRY(dev_alloc(dcount, 1));
+    HIP_TRY(launch_fill_u32(dcount.p, 1, 0, ctx->stream));
+    HIP_TRY(launch_trace_pairs(ctx->state(), ctx->cur, world, w->n, (uint32_t)(w->t - 1), drec.p, pair_capacity,
+                               dcount.p, ctx->stream));
+    uint32_t count = 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(&count, dcount.p, sizeof(count), hipMemcpyDeviceToHost));
+    if (n_pairs) *n_pairs = count;
+    const uint32_t got = std::min(count, pair_capacity);
+    if (got) {
+        HIP_TRY(hipMemcpy(pairs, drec.p, got * sizeof(pdpd_pair_record), hipMemcpyDeviceToHost));
+        std::sort(pairs, pairs + got, [](const pdpd_pair_record &a, const pdpd_pair_record &b) {
+            return a.home != b.home ? a.home < b.home : a.other < b.other;
+        });
+    }
+    drec.release();
+    dcount.release();
+    // Hookean bonds: positions and hashes from the world's buffers, forces as the step evaluated them
+    const uint32_t nbond = (uint32_t)w->bonds.size();
+    if (n_bonds) *n_bonds = nbond;
+    if (bond_capacity && nbond) {
+        std::vector<float4> P, V, F;
+        if ((rc = fetch_world(ctx, *w, P, V, F))) return rc;
+        std::vector<uint32_t> slot_of(w->n);
+        for (uint32_t s = 0; s < w->n; s++) {
+            const uint32_t i = f2u(V[s].w) - w->bead_off;
+            if (i >= w->n) return fail(PDPD_ERR_STATE, "corrupt bead index in device state");
+            slot_of[i] = s;
+        }
+        const uint32_t nb = std::min(nbond, bond_capacity);
+        std::vector<double> tf((size_t)nb * kTermFrcStride);
+        HIP_TRY(hipMemcpy(tf.data(), ctx->term_frc.p + (size_t)w->term_base * kTermFrcStride,
+                          tf.size() * sizeof(double), hipMemcpyDeviceToHost));
+        auto min_image = [](double d, double len) { return d > 0.5 * len ? d - len : (d < -0.5 * len ? d + len : d); };
+        for (uint32_t j = 0; j < nb; j++) {
+            const float4 a = P[slot_of[w->bonds[j].a]], b = P[slot_of[w->bonds[j].b]];
+            pdpd_bond_record &r = bonds[j];
+            r.head = f2u(a.w);
+            r.tail = f2u(b.w);
+            const double da[3] = {a.x, a.y, a.z}, db[3] = {b.x, b.y, b.z};
+            for (int d = 0; d < 3; d++) r.dx[d] = min_image(db[d] - da[d], (double)(float)w->L[d]);
+            r.dr = std::sqrt(std::fma(r.dx[2], r.dx[2], std::fma(r.dx[1], r.dx[1], r.dx[0] * r.dx[0])));
+            for (int d = 0; d < 3; d++) r.f_head[d] = -tf[(size_t)j * kTermFrcStride + d];
+        }
+    }
+    if (count > pair_capacity || nbond > bond_capacity) {
+        if (pair_capacity == 0 && bond_capacity == 0) return PDPD_OK;    // a count query
+        return fail(PDPD_ERR_INVALID, "force trace: record capacity too small");
+    }
+    return PDPD_OK;
 }
 
 // Times launch shapes that do not change results (the pair-force grid and the
@@ -1034,7 +1112,8 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     (void)hipEventDestroy(e1);
     if ((rc = restore())) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    sp.release(); sv.release(); sf.release();
+    sp.release(); sv.release(); sf.release(); ss.release();
+    ctx->steps_on_layout = 0;      // the cell lists are the last trial's, not the restored state's
     ctx->force_grid = best_grid;
     ctx->force_variant = best_var;
     ctx->graph_steps = best_graph;

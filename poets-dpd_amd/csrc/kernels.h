@@ -2,6 +2,7 @@
 #pragma once
 
 #include "device_types.h"
+#include "pdpd_hip.h"
 
 namespace pdpd {
 
@@ -37,6 +38,9 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
 hipError_t launch_force(const StepLaunch &L, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
+// Pair records of world `world` (n_beads beads) from buffer buf, step counter t; appended at *count.
+hipError_t launch_trace_pairs(const DeviceState &S, int buf, uint32_t world, uint32_t n_beads, uint32_t t,
+                              pdpd_pair_record *out, uint32_t cap, uint32_t *count, hipStream_t st);
 // slot_of_bead of a world moved from bead offset old_off to new_off (old = null: identity).
 hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
                                hipStream_t st);

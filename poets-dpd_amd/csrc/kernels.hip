@@ -833,6 +833,81 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
     }
 }
 
+// Force trace (debugging): every pair the pair-force kernel evaluates for one
+// world, recomputed from the buffers the step left (x(t+dt) and the half-step
+// velocities the step used) by one thread per home bead over its 27 cells, with
+// the acceptance test and the arithmetic of k_force, plus the per-term
+// quantities the reference logs. Records are appended in any order.
+__global__ __launch_bounds__(kBeadThreads) void k_trace_pairs(DeviceState S, int buf, uint32_t w, uint32_t t,
+                                                              pdpd_pair_record *__restrict__ out, uint32_t cap,
+                                                              uint32_t *__restrict__ count)
+{
+    const WorldDev *__restrict__ W = &S.worlds[w];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W->n_beads) return;
+    const uint32_t s = W->bead_off + i;
+    const float4 *__restrict__ pos = S.pos[buf];
+    const float4 *__restrict__ vel = S.vel[buf];
+    const float4 hp = pos[s], hv = vel[s];
+    const uint32_t hh = fbits(hp.w);
+    if ((W->stationary >> (hh >> 28)) & 1u) return;
+    const uint64_t th = step_hash(t, W->seed_lo, W->seed_hi);
+    const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
+    const float scale = W->rng_scale;
+    const int L[3] = {W->L[0], W->L[1], W->L[2]};
+    const int c[3] = {cell_coord(hp.x, L[0]), cell_coord(hp.y, L[1]), cell_coord(hp.z, L[2])};
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++)
+            for (int oz = -1; oz <= 1; oz++) {
+                const int o[3] = {ox, oy, oz};
+                int n[3];
+                float sh[3];
+                for (int d = 0; d < 3; d++) {
+                    n[d] = c[d] + o[d];
+                    uint32_t code = 0;
+                    if (n[d] < 0) { n[d] += L[d]; code = 2u; } else if (n[d] >= L[d]) { n[d] -= L[d]; code = 1u; }
+                    sh[d] = shift_of(code, W->Lf[d]);
+                }
+                const uint32_t cell = W->cell_off + (uint32_t)((n[0] * L[1] + n[1]) * L[2] + n[2]);
+                for (uint32_t j = S.cell_start[cell]; j < S.cell_start[cell + 1]; j++) {
+                    const float4 op = pos[j];
+                    const float dx = hp.x - (op.x + sh[0]), dy = hp.y - (op.y + sh[1]), dz = hp.z - (op.z + sh[2]);
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    if (!(r2 < 1.0f && r2 >= kMinR2)) continue;
+                    const float4 ov = vel[j];
+                    const uint32_t oh = fbits(op.w);
+                    const float rr = sqrtf(r2);
+                    const float inv_r = 1.0f / rr;
+                    const float wr = 1.0f - rr;
+                    const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+                    const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
+                    const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+                    const uint32_t ti = (hh >> 28) * kMaxTypes + (oh >> 28);
+                    const float a = W->a[ti], sg = W->sg[ti];
+                    const float sgw = sg * wr;
+                    const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
+                    const float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                    pdpd_pair_record r;
+                    r.home = hh;
+                    r.other = oh;
+                    r.dx[0] = dx; r.dx[1] = dy; r.dx[2] = dz;
+                    r.dr = rr;
+                    r.dv[0] = dvx; r.dv[1] = dvy; r.dv[2] = dvz;
+                    r.diss_strength = sg * sg;
+                    r.inv_root_dt = scale;
+                    r.gammap = sgw * sgw;
+                    r.rng = u;
+                    r.con = a * wr;
+                    r.diss = -(sgw * sgw) * rdotv;
+                    r.rng_scale = sgw * scale;
+                    r.rand = r.rng_scale * u;
+                    r.f[0] = ex * F; r.f[1] = ey * F; r.f[2] = ez * F;
+                    const uint32_t k = atomicAdd(count, 1u);
+                    if (k < cap) out[k] = r;
+                }
+            }
+}
+
 // After the pair forces have read them: beads of stationary types end the step
 // with v = 0 (naive_dpd_engine_half_merge.hpp:387-390). Runs only when such
 // beads are resident.
@@ -931,6 +1006,14 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     k_fill_u32<<<ceil_div(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_pairs(const DeviceState &S, int buf, uint32_t world, uint32_t n_beads, uint32_t t,
+                              pdpd_pair_record *out, uint32_t cap, uint32_t *count, hipStream_t st)
+{
+    if (n_beads == 0) return hipSuccess;
+    k_trace_pairs<<<ceil_div(n_beads, kBeadThreads), kBeadThreads, 0, st>>>(S, buf, world, t, out, cap, count);
     return hipGetLastError();
 }
 

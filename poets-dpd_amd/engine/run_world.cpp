@@ -16,7 +16,10 @@
 // bead_steps_per_s_so_far,bead_steps_per_s_interval" (src/run_world.cpp:181-185).
 // The world is checked with validate_world_state(state, GetMaxBondLength())
 // before it is attached, as src/run_world.cpp:155-157 does. States are written
-// on host threads while the engine runs the next interval.
+// on host threads while the engine runs the next interval. With PDPD_LOG=path
+// in the environment every step's forces are logged to path in the reference's
+// force-log format (src/run_world.cpp:120-125; the engine then runs one step
+// at a time).
 #include "pdpd/hip_dpd_engine.hpp"
 #include "pdpd/world_state_io.hpp"
 #include "pdpd/world_state_validate.hpp"
@@ -45,6 +48,7 @@ static int usage()
     fprintf(stderr, "   --text-state : write the states as text v0 with 17 significant digits instead of v2binary.\n");
     fprintf(stderr, "  engine names:\n");
     for (const auto &n : DPDEngineFactory::ListFactories()) fprintf(stderr, "    %s\n", n.c_str());
+    fprintf(stderr, "  env:\n     PDPD_LOG=log-path : do full force logging to given file.\n");
     fprintf(stderr, "    Either snapshot_interval_size must divide state_interval_size, or vice-versa\n");
     return 1;
 }
@@ -67,6 +71,15 @@ int main(int argc, const char *argv[])
         }
     }
     if (args.size() < 3) return usage();
+    std::unique_ptr<FileLogger> logger;
+    if (const char *dst = getenv("PDPD_LOG")) {
+        logger.reset(new FileLogger(dst));
+        ForceLogging::set_logger(logger.get());
+        fprintf(stderr, "Logging to %s\n", dst);
+    }
+    struct Unset {
+        ~Unset() { ForceLogging::set_logger(nullptr); }
+    } unset;
     try {
         const std::string engine_name = args[0], src = args[1], base = args[2];
         const int interval_count = args.size() > 3 ? std::stoi(args[3]) : 1000;

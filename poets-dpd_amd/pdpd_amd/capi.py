@@ -84,6 +84,13 @@ INTERVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int64
                           C.c_void_p, C.c_void_p, C.c_void_p)
 
 # Every symbol include/pdpd_hip.h declares, with its signature.
+# pdpd_pair_record / pdpd_bond_record of pdpd_hip.h
+PAIR_RECORD = np.dtype([("home", "<u4"), ("other", "<u4"), ("dx", "<f4", 3), ("dr", "<f4"), ("dv", "<f4", 3),
+                        ("diss_strength", "<f4"), ("inv_root_dt", "<f4"), ("gammap", "<f4"), ("rng", "<f4"),
+                        ("con", "<f4"), ("diss", "<f4"), ("rng_scale", "<f4"), ("rand", "<f4"), ("f", "<f4", 3)])
+BOND_RECORD = np.dtype([("head", "<u4"), ("tail", "<u4"), ("dx", "<f8", 3), ("dr", "<f8"), ("f_head", "<f8", 3)])
+assert PAIR_RECORD.itemsize == 80 and BOND_RECORD.itemsize == 64
+
 SIGNATURES = {
     "pdpd_last_error": (C.c_char_p, []),
     "pdpd_abi_version": (C.c_int, []),
@@ -120,6 +127,8 @@ SIGNATURES = {
                                      C.c_void_p]),
     "pdpd_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "pdpd_time_force": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
+    "pdpd_trace_forces": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                    C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -333,6 +342,17 @@ class Context:
         ms = C.c_double()
         check(self.lib.pdpd_time_force(self.h, int(reps), C.byref(ms)))
         return ms.value
+
+    def trace_forces(self, slot):
+        """Force trace of the step that produced the world's current state
+        (pdpd_trace_forces): (pair records, bond records) as numpy record arrays."""
+        npair, nbond = C.c_uint32(), C.c_uint32()
+        check(self.lib.pdpd_trace_forces(self.h, slot, None, 0, C.byref(npair), None, 0, C.byref(nbond)))
+        pairs = np.zeros(npair.value, PAIR_RECORD)
+        bonds = np.zeros(nbond.value, BOND_RECORD)
+        check(self.lib.pdpd_trace_forces(self.h, slot, pairs.ctypes.data, len(pairs), C.byref(npair),
+                                         bonds.ctypes.data, len(bonds), C.byref(nbond)))
+        return pairs, bonds
 
     def autotune(self, steps_per_trial=20):
         buf = C.create_string_buffer(256)

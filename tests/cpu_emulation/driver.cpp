@@ -126,6 +126,15 @@ int main(int argc, char **argv)
             CHECK(pdpd_run(ctx, c.steps));
             CHECK(pdpd_sync(ctx));
         }
+        {   // force trace of the last step: counts, then the records
+            uint32_t np = 0, nbd = 0;
+            CHECK(pdpd_trace_forces(ctx, slot, nullptr, 0, &np, nullptr, 0, &nbd));
+            std::vector<pdpd_pair_record> pr(np);
+            std::vector<pdpd_bond_record> br(nbd);
+            CHECK(pdpd_trace_forces(ctx, slot, pr.data(), np, &np, br.data(), nbd, &nbd));
+            if (np == 0 || (c.bonded > 0 && nbd == 0)) { fprintf(stderr, "empty trace\n"); return 1; }
+            fprintf(stderr, "  trace %u pairs %u bonds\n", np, nbd);
+        }
         std::vector<double> x(3 * w.d.n_beads), v(x.size()), f(x.size());
         int64_t t;
         CHECK(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t));
