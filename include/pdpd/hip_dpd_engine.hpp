@@ -39,6 +39,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace pdpd_host {
@@ -49,20 +50,52 @@ inline void check(int rc, const char *what)
 }
 
 // WorldState -> pdpd_world_desc. Owns the flattened arrays the descriptor
-// points into; valid while this object lives.
+// points into; valid while this object lives. DPD strengths that are
+// expressions of the cell position are evaluated at every cell with the
+// bindings of naive_dpd_engine.hpp:158-183 (x, y, z the cell coordinates, ux,
+// uy, uz = (coordinate + 0.5) / cells along the axis), unless cell_tables is
+// false (the support check, which needs only the constant part).
 struct WorldDescBuffers {
     pdpd_world_desc desc{};
     std::vector<double> cons, diss, x, v, f, bond_kappa, bond_r0, angle_kappa, angle_theta0;
+    std::vector<double> cell_cons, cell_diss;
     std::vector<uint8_t> stationary, bead_type, is_monomer, polymer_offset;
     std::vector<uint32_t> polymer_id, pt_bond_start, bond_head, bond_tail, pt_angle_start, angle_head_bond,
         angle_tail_bond, polymer_type, polymer_bead_start, polymer_bead_ids;
 
-    explicit WorldDescBuffers(const WorldState &s)
+    explicit WorldDescBuffers(const WorldState &s, bool cell_tables = true)
     {
         const size_t nt = s.bead_types.size();
+        bool spatial = false;
         for (size_t i = 0; i < s.interactions.size(); i++) {
-            cons.push_back((double)s.interactions[i].conservative);
-            diss.push_back((double)s.interactions[i].dissipative);
+            const auto &ii = s.interactions[i];
+            spatial |= !ii.conservative.is_constant() || !ii.dissipative.is_constant();
+            cons.push_back(ii.conservative.is_constant() ? (double)ii.conservative : 0.0);
+            diss.push_back(ii.dissipative.is_constant() ? (double)ii.dissipative : 0.0);
+        }
+        bool box_ok = true;
+        for (int k = 0; k < 3; k++) box_ok &= s.box[k] >= 1 && s.box[k] <= 4096 && s.box[k] == (double)(int)s.box[k];
+        if (spatial && cell_tables && box_ok && s.interactions.size() == nt * nt) {
+            const int L[3] = {(int)s.box[0], (int)s.box[1], (int)s.box[2]};
+            const size_t m = nt * nt;
+            cell_cons.resize((size_t)L[0] * L[1] * L[2] * m);
+            cell_diss.resize(cell_cons.size());
+            std::unordered_map<std::string, double> b;
+            size_t c = 0;
+            for (int cx = 0; cx < L[0]; cx++)
+                for (int cy = 0; cy < L[1]; cy++)
+                    for (int cz = 0; cz < L[2]; cz++, c++) {
+                        b["x"] = cx; b["y"] = cy; b["z"] = cz;
+                        b["ux"] = (cx + 0.5) / L[0];
+                        b["uy"] = (cy + 0.5) / L[1];
+                        b["uz"] = (cz + 0.5) / L[2];
+                        for (size_t k = 0; k < m; k++) {
+                            auto pc = s.interactions[k].conservative;     // (evaluate is not const in every Parameter)
+                            auto pd = s.interactions[k].dissipative;
+                            cell_cons[c * m + k] = pc.evaluate(b);
+                            cell_diss[c * m + k] = pd.evaluate(b);
+                        }
+                    }
         }
         bool any_stationary = false;
         for (const auto &bt : s.bead_types) {
@@ -131,6 +164,8 @@ struct WorldDescBuffers {
         d.polymer_type = polymer_type.data();
         d.polymer_bead_start = polymer_bead_start.data();
         d.polymer_bead_ids = polymer_bead_ids.data();
+        d.cell_conservative = cell_cons.empty() ? nullptr : cell_cons.data();
+        d.cell_dissipative = cell_diss.empty() ? nullptr : cell_diss.data();
     }
 };
 
@@ -202,6 +237,7 @@ protected:
     bool CanSupportHookeanBonds() const override { return true; }
     bool CanSupportAngleBonds() const override { return true; }
     bool CanSupportStationaryBeadTypes() const override { return true; }
+    bool CanSupportSpatialDPDParameters() const override { return true; }
 
 public:
     explicit HipDPDEngine(int device = 0, unsigned graph_steps = 20) : m_ctx(device), m_graph_steps(graph_steps) {}
@@ -215,7 +251,7 @@ public:
     {
         std::string r = DPDEngine::CanSupport(s);
         if (!r.empty()) return r;
-        pdpd_host::WorldDescBuffers b(*s);
+        pdpd_host::WorldDescBuffers b(*s, false);
         return pdpd_can_support(&b.desc);
     }
 

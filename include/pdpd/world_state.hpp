@@ -4,10 +4,12 @@
 // Names, fields and meanings follow include/dpd/core/dpd_state.hpp:22-264 and
 // include/dpd/core/vec3.hpp of poets-dpd, because code written against that
 // interface (tools, tests, the HIP engine) must compile against either copy.
-// Parameters are constants only: spatially varying expressions
-// (include/dpd/core/expression.hpp:328-438) are not part of this data model and
-// an engine that meets one through the reference headers rejects it in CanSupport.
+// Parameters are constants or expressions (include/dpd/core/expression.hpp;
+// pdpd/expression.hpp here); the HIP engine supports DPD strengths that vary
+// with the cell position (variables x, y, z, ux, uy, uz) and rejects others.
 #pragma once
+
+#include "pdpd/expression.hpp"
 
 #include <array>
 #include <cassert>
@@ -62,17 +64,37 @@ using vec3f_t = vec3g_t<float>;
 using vec3i_t = vec3g_t<int32_t>;
 
 // A constant parameter (the constant case of expression.hpp's Parameter).
+// A constant, or an expression of variables (pdpd/expression.hpp; the
+// reference's Parameter, expression.hpp:328-431). Using an expression as a
+// constant throws; engines evaluate spatial ones per cell.
 struct Parameter {
     double m_value = 0.0;
+    pdpd_expr::NodePtr m_expr;
     Parameter() = default;
     Parameter(double v) : m_value(v) {}
-    Parameter &operator=(double v) { m_value = v; return *this; }
-    operator double() const { return m_value; }
-    bool operator==(const Parameter &o) const { return m_value == o.m_value; }
-    bool is_constant() const { return true; }
-    void collect_variables(std::unordered_set<std::string> &) const {}
-    double evaluate(const std::unordered_map<std::string, double> &) const { return m_value; }
-    std::string as_string() const { return std::to_string(m_value); }
+    explicit Parameter(const std::string &src) { set(pdpd_expr::parse(src)); }
+    Parameter &operator=(double v) { m_value = v; m_expr.reset(); return *this; }
+    operator double() const
+    {
+        if (m_expr)
+            throw std::runtime_error("Attempt to use expression with variables as a constant. Expr=." + m_expr->text());
+        return m_value;
+    }
+    bool operator==(const Parameter &o) const
+    {
+        return m_expr ? (o.m_expr && m_expr->text() == o.m_expr->text()) : (!o.m_expr && m_value == o.m_value);
+    }
+    bool is_constant() const { return !m_expr; }
+    void collect_variables(std::unordered_set<std::string> &out) const { if (m_expr) m_expr->variables(out); }
+    double evaluate(const std::unordered_map<std::string, double> &b) const { return m_expr ? m_expr->eval(b) : m_value; }
+    std::string as_string() const { return m_expr ? m_expr->text() : std::to_string(m_value); }
+
+private:
+    void set(pdpd_expr::NodePtr e)
+    {
+        if (e->op == 'c') m_value = e->value;
+        else m_expr = std::move(e), m_value = 0.0;
+    }
 };
 
 // 32-bit bead identity (dpd_state.hpp:18-35):

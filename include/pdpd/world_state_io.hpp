@@ -22,8 +22,9 @@
 //
 // Lines whose first word starts with '#' and empty lines are skipped. Bead
 // positions are kept relative to the origin; text files hold them with the
-// origin added, binary files without. Parameters are constants (expressions are
-// rejected with a message). ".gz" and ".bz2" files go through gzip / bzip2.
+// origin added, binary files without. Parameters are numbers or expressions
+// (pdpd/expression.hpp; no spaces inside, as fields are split on spaces).
+// ".gz" and ".bz2" files go through gzip / bzip2.
 //
 // Binary polymer record (little-endian):
 //   u16  bits 0-12 polymer type; bit 13: polymer id is not previous + 1;
@@ -69,6 +70,21 @@ struct Fields {
     {
         if (i >= parts.size()) throw std::runtime_error("missing field " + std::to_string(i) + " at line " + std::to_string(line_no));
         return parts[i];
+    }
+    // A number (strtod, so also a negative one, which the expression language
+    // has no literal for), else an expression.
+    Parameter param(size_t i) const
+    {
+        const std::string &t = at(i);
+        char *e = nullptr;
+        const double v = std::strtod(t.c_str(), &e);
+        if (!t.empty() && e && *e == 0) return Parameter(v);
+        try {
+            return Parameter(t);
+        } catch (const std::runtime_error &e) {
+            throw std::runtime_error(std::string(e.what()) + " (parameter '" + at(i) + "' at line " +
+                                     std::to_string(line_no) + ")");
+        }
     }
     double real(size_t i) const
     {
@@ -120,8 +136,9 @@ inline Fields expect(std::istream &src, const char *prefix, int n, int &line_no)
     throw std::runtime_error("expected '" + std::string(prefix) + "' but reached the end of the file");
 }
 
-// The reference prints a constant parameter with std::to_string (expression.hpp).
-inline std::string param(double v) { return std::to_string(v); }
+// The reference prints a parameter as its expression text, a constant with
+// std::to_string (expression.hpp:431-435).
+inline std::string param(const Parameter &p) { return p.as_string(); }
 
 // ---- binary records -------------------------------------------------------
 
@@ -380,11 +397,11 @@ inline WorldState read_world_state(std::istream &src)
     s.interactions.assign(nt * nt, InteractionStrength{});
     for (size_t i = 0; i < nt; i++) {
         f = expect(src, "ConservativeStrength", (int)nt + 2, ln);
-        for (size_t j = 0; j < nt; j++) s.interactions[i * nt + j].conservative = f.real(j + 2);
+        for (size_t j = 0; j < nt; j++) s.interactions[i * nt + j].conservative = f.param(j + 2);
     }
     for (size_t i = 0; i < nt; i++) {
         f = expect(src, "DissipativeStrength", (int)nt + 2, ln);
-        for (size_t j = 0; j < nt; j++) s.interactions[i * nt + j].dissipative = f.real(j + 2);
+        for (size_t j = 0; j < nt; j++) s.interactions[i * nt + j].dissipative = f.param(j + 2);
     }
     s.bead_types.resize(nt);
     for (size_t i = 0; i < nt; i++) {
@@ -423,8 +440,8 @@ inline WorldState read_world_state(std::istream &src)
             Fields g = expect(src, "Bond", 5, ln);
             b.bead_offset_head = (uint32_t)g.uint(1);
             b.bead_offset_tail = (uint32_t)g.uint(2);
-            b.kappa = g.real(3);
-            b.r0 = g.real(4);
+            b.kappa = g.param(3);
+            b.r0 = g.param(4);
             if (b.bead_offset_head >= p.bead_types.size() || b.bead_offset_tail >= p.bead_types.size())
                 throw std::runtime_error("bond bead offset out of range at line " + std::to_string(ln));
         }
@@ -432,8 +449,8 @@ inline WorldState read_world_state(std::istream &src)
             Fields g = expect(src, "BondPair", 5, ln);
             b.bond_offset_head = (uint32_t)g.uint(1);
             b.bond_offset_tail = (uint32_t)g.uint(2);
-            b.kappa = g.real(3);
-            b.theta0 = g.real(4);
+            b.kappa = g.param(3);
+            b.theta0 = g.param(4);
             if (b.bond_offset_head >= p.bonds.size() || b.bond_offset_tail >= p.bonds.size())
                 throw std::runtime_error("bond pair offset out of range at line " + std::to_string(ln));
         }

@@ -43,11 +43,16 @@ inline void validate_world_state(const WorldState &s, double max_r = 1.0)
     need(s.interactions.size() == nt * nt, "interactions has bead_types^2 entries");
     for (size_t i = 0; i < nt; i++)
         for (size_t j = 0; j < nt; j++) {
+            // constant strengths only; expressions are evaluated per cell by the engine
             const InteractionStrength &a = s.interactions[i * nt + j], &b = s.interactions[j * nt + i];
-            need((double)a.conservative == (double)b.conservative, "conservative strengths are symmetric");
-            need((double)a.dissipative == (double)b.dissipative, "dissipative strengths are symmetric");
-            need(0 <= (double)a.conservative, "conservative strengths are >= 0");
-            need(0 <= (double)a.dissipative, "dissipative strengths are >= 0");
+            if (a.conservative.is_constant()) {
+                need(a.conservative == b.conservative, "conservative strengths are symmetric");
+                need(0 <= (double)a.conservative, "conservative strengths are >= 0");
+            }
+            if (a.dissipative.is_constant()) {
+                need(a.dissipative == b.dissipative, "dissipative strengths are symmetric");
+                need(0 <= (double)a.dissipative, "dissipative strengths are >= 0");
+            }
         }
 
     names.clear();

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PDPD_ABI_VERSION 1
+#define PDPD_ABI_VERSION 2
 
 enum {
     PDPD_OK = 0,
@@ -86,6 +86,14 @@ typedef struct pdpd_world_desc {
     const uint32_t *polymer_type;  /* Polymer::polymer_type                          */
     const uint32_t *polymer_bead_start; /* n_polymers+1 -> polymer_bead_ids         */
     const uint32_t *polymer_bead_ids;   /* Polymer::bead_ids                         */
+
+    /* Spatially varying DPD strengths (naive_dpd_engine.hpp:145-188): the
+     * interactions evaluated at every cell, n_cells * n_bead_types^2 each, cell
+     * index (x * Ly + y) * Lz + z, or NULL for constant strengths (then the
+     * arrays above are used). A pair uses the mean of its two beads' cells
+     * (naive_dpd_engine.hpp:452-468). */
+    const double *cell_conservative;
+    const double *cell_dissipative;
 } pdpd_world_desc;
 
 typedef struct pdpd_limits {

@@ -483,9 +483,20 @@ void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, 
                 float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
                 uint32_t oh = w->hash[j];
                 int ti = htype * nt + (int)(oh >> 28);
-                float sgw = w->sg[ti] * wr;
+                float a, sg;
+                if (w->cell_par) {
+                    const int hcl = (hc[0] * w->L[1] + hc[1]) * w->L[2] + hc[2];
+                    const float *ph = w->cell_par + 2 * ((size_t)hcl * nt * nt + ti);
+                    const float *po = w->cell_par + 2 * ((size_t)oc * nt * nt + ti);
+                    a = (ph[0] + po[0]) * 0.5f;
+                    sg = sqrtf((ph[1] + po[1]) * 0.5f);
+                } else {
+                    a = w->a[ti];
+                    sg = w->sg[ti];
+                }
+                float sgw = sg * wr;
                 float u = orc_default_hash(t_hash, hh, oh);
-                float F = fmaf(w->a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, scale * u));
                 fx = fmaf(ex, F, fx);
                 fy = fmaf(ey, F, fy);
                 fz = fmaf(ez, F, fz);

@@ -118,6 +118,12 @@ typedef struct orc_f32_world {
     const int32_t *angle_head, *angle_mid, *angle_tail;
     const double *angle_kappa, *angle_cos0, *angle_sin0;
     const uint8_t *stationary;             /* n_types, may be NULL */
+    /* Spatially varying strengths (naive_dpd_engine.hpp:145-188, 452-468), or
+     * NULL: per cell (x * L1 + y) * L2 + z and ordered type pair
+     * (home * n_types + other), the pair (a, gamma) in FP32. A pair then uses
+     * a = (a_home_cell + a_other_cell) * 0.5f and
+     * sqrt(gamma) = sqrtf((g_home_cell + g_other_cell) * 0.5f). */
+    const float *cell_par;
 } orc_f32_world;
 
 int orc_f32_step(orc_f32_world *w, int32_t *order, int32_t *cell_start, int32_t *key, float *fabs_sum);

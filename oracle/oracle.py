@@ -39,7 +39,7 @@ class OrcF32World(C.Structure):
         ("bond_head", C.c_void_p), ("bond_tail", C.c_void_p), ("bond_kappa", C.c_void_p), ("bond_r0", C.c_void_p),
         ("angle_head", C.c_void_p), ("angle_mid", C.c_void_p), ("angle_tail", C.c_void_p),
         ("angle_kappa", C.c_void_p), ("angle_cos0", C.c_void_p), ("angle_sin0", C.c_void_p),
-        ("stationary", C.c_void_p),
+        ("stationary", C.c_void_p), ("cell_par", C.c_void_p),
     ]
 
 
@@ -111,6 +111,8 @@ class F64State:
         self.v = np.ascontiguousarray(world.v, dtype=np.float64).copy()
         self.f = np.ascontiguousarray(world.f, dtype=np.float64).copy()
         self.t = int(world.t)
+        if getattr(world, "spatial", False):
+            raise NotImplementedError("the FP64 oracle takes constant strengths only (FP32 oracle: per-cell strengths)")
         topo = world.flat_topology()
         self.keep = dict(topo)
         self.keep["cons"] = np.ascontiguousarray(world.conservative.reshape(-1), np.float64)
@@ -192,6 +194,12 @@ class F32State:
         self.keep["hash"] = np.ascontiguousarray(world.hash, np.uint32)
         self.keep["stat"] = (np.ascontiguousarray(world.stationary, np.uint8)
                              if world.stationary is not None else None)
+        # strengths per cell (the world's tables rounded to FP32, (a, gamma) per
+        # cell and ordered type pair), or None for constant strengths
+        self.keep["cell_par"] = None
+        if getattr(world, "spatial", False):
+            cc, cd = world.cell_strengths()
+            self.keep["cell_par"] = np.ascontiguousarray(np.stack([cc, cd], axis=-1).reshape(-1), np.float32)
         self.order = None
         self.cell_start = None
         self.key = None
@@ -216,6 +224,7 @@ class F32State:
                      "angle_head", "angle_mid", "angle_tail", "angle_kappa", "angle_cos0", "angle_sin0"):
             setattr(s, name, _p(k[name]))
         s.stationary = _p(k["stat"])
+        s.cell_par = _p(k["cell_par"])
         return s
 
     def step(self, nsteps=1):
@@ -266,6 +275,9 @@ def ref_lib(fast=False):
         L.ref_run.restype = C.c_int
         L.ref_run.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.ref_run_file.restype = C.c_int
+        L.ref_run_file.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_int64)]
         L.ref_run_logged.restype = C.c_int
         L.ref_run_logged.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]
         L.ref_force_log_sample.restype = C.c_int
@@ -291,6 +303,20 @@ def ref_run(world, engine, nsteps, fast=False, timing=None):
     rc = L.ref_run(C.addressof(d), engine.encode(), int(nsteps), _p(x), _p(v), _p(f), C.byref(t), C.byref(secs))
     if timing is not None:
         timing["run_s"] = secs.value
+    if rc == 1:
+        raise NotImplementedError(L.ref_last_error().decode())
+    if rc:
+        raise RuntimeError(L.ref_last_error().decode())
+    return x, v, f, t.value
+
+
+def ref_run_file(path, n_beads, engine, nsteps):
+    """Run a reference engine (naive | naive_core) on the world the
+    reference's own reader makes of a state file; returns (x, v, f, t)."""
+    x = np.zeros((n_beads, 3)); v = np.zeros((n_beads, 3)); f = np.zeros((n_beads, 3))
+    t = C.c_int64()
+    L = ref_lib()
+    rc = L.ref_run_file(str(path).encode(), engine.encode(), int(nsteps), _p(x), _p(v), _p(f), C.byref(t))
     if rc == 1:
         raise NotImplementedError(L.ref_last_error().decode())
     if rc:

@@ -14,6 +14,7 @@
 #include "dpd/engines/naive/naive_dpd_engine_half_step.hpp"
 #include "dpd/core/logging.hpp"
 #include "dpd/core/logging_impl.hpp"
+#include "dpd/core/dpd_state_io.hpp"
 
 #include "pdpd_hip.h"
 
@@ -153,6 +154,40 @@ int ref_run(const pdpd_world_desc *d, const char *engine, int nsteps, double *x,
                 if (f) f[3 * i + k] = s.beads[i].f[k];
             }
         }
+        if (t_out) *t_out = s.t;
+        e->Attach(nullptr);
+        return 0;
+    } catch (std::exception &ex) {
+        g_err = ex.what();
+        return -1;
+    } catch (std::string &sx) {
+        g_err = sx;
+        return -1;
+    }
+}
+
+// A run of a world read from a state file by the reference's own reader
+// (dpd_state_io.hpp), so parameters given as expressions reach the engine as
+// the reference parses them. x, v, f: 3 * beads, bead-index order.
+int ref_run_file(const char *path, const char *engine, int nsteps, double *x, double *v, double *f, int64_t *t_out)
+{
+    try {
+        WorldState s = read_world_state(std::string(path));
+        std::unique_ptr<DPDEngine> e;
+        std::string name(engine);
+        if (name == "naive") e.reset(new NaiveDPDEngine<false>());
+        else if (name == "naive_core") e.reset(new NaiveDPDEngine<true>());
+        else { g_err = "unknown engine " + name; return -1; }
+        std::string why = e->CanSupport(&s);
+        if (!why.empty()) { g_err = why; return 1; }
+        e->Attach(&s);
+        if (nsteps > 0) e->Run((unsigned)nsteps);
+        for (size_t i = 0; i < s.beads.size(); i++)
+            for (int k = 0; k < 3; k++) {
+                if (x) x[3 * i + k] = s.beads[i].x[k];
+                if (v) v[3 * i + k] = s.beads[i].v[k];
+                if (f) f[3 * i + k] = s.beads[i].f[k];
+            }
         if (t_out) *t_out = s.t;
         e->Attach(nullptr);
         return 0;

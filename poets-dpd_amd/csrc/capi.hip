@@ -81,6 +81,7 @@ struct WorldHost {
     uint32_t stationary = 0;      // bit per stationary bead type
     float a[kMaxTypes * kMaxTypes] = {};
     float sg[kMaxTypes * kMaxTypes] = {};
+    std::vector<float2> cell_par;                  // per cell and type pair (a, gamma), or empty: constant strengths
     std::vector<float4> up_pos, up_vel, up_frc;   // pending upload (bead order)
     std::vector<TermRec> bonds, angles;            // local bead indices, reference order
     std::vector<uint32_t> stationary_beads;        // local indices of beads of stationary types
@@ -145,6 +146,8 @@ struct pdpd_ctx {
     DevBuf<TermRec> terms;
     DevBuf<double> term_frc;
     DevBuf<StationaryBead> stationary;
+    DevBuf<float2> cell_par;
+    bool spatial = false;                 // some resident world has per-cell strengths
     int cur = 0;
     uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0, n_terms = 0;
     std::vector<int> order;               // slots in layout order
@@ -173,6 +176,7 @@ struct pdpd_ctx {
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
+        cell_par.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -187,6 +191,7 @@ struct pdpd_ctx {
         S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
         S.term_start = term_start.p; S.bead_terms = bead_terms.p; S.terms = terms.p; S.term_frc = term_frc.p;
         S.stationary_beads = stationary.p;
+        S.cell_par = cell_par.p;
         return S;
     }
 
@@ -203,6 +208,7 @@ struct pdpd_ctx {
         L.force_variant = force_variant;
         L.tile_sum = tile_sum.p;
         L.lookback_tiles = lookback_tiles;
+        L.spatial = spatial ? 1u : 0u;
         return L;
     }
 };
@@ -321,8 +327,9 @@ int relayout(pdpd_ctx *ctx)
     if (!ctx->scan_ticket.p) HIP_TRY(dev_alloc(ctx->scan_ticket, 1));
     HIP_TRY(launch_fill_u32(ctx->cell_count.p, (uint32_t)nc, 0, ctx->stream));
 
-    // world table
+    // world table; per-cell strengths of the worlds that have them, concatenated
     std::vector<WorldDev> wd(order.size());
+    std::vector<float2> cpar;
     for (size_t i = 0; i < order.size(); i++) {
         const WorldHost &w = *ws[i];
         WorldDev &d = wd[i];
@@ -338,8 +345,16 @@ int relayout(pdpd_ctx *ctx)
         d.stationary = w.stationary;
         memcpy(d.a, w.a, sizeof(d.a));
         memcpy(d.sg, w.sg, sizeof(d.sg));
+        if (!w.cell_par.empty()) {
+            if (cpar.size() + w.cell_par.size() > UINT32_MAX) return fail(PDPD_ERR_INVALID, "per-cell strength tables too large");
+            d.spatial = 1;
+            d.cell_par_off = (uint32_t)cpar.size();
+            cpar.insert(cpar.end(), w.cell_par.begin(), w.cell_par.end());
+        }
     }
     HIP_TRY(dev_upload(ctx->wdev, wd));
+    HIP_TRY(dev_upload(ctx->cell_par, cpar));
+    ctx->spatial = !cpar.empty();
     HIP_TRY(dev_alloc(ctx->ctr, 2 * std::max<size_t>(1, order.size())));
     ctx->ctr_host.assign(2 * std::max<size_t>(1, order.size()), WorldCtr{0, 0});
 
@@ -616,6 +631,20 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
             w.a[i * kMaxTypes + j] = (float)d->conservative[i * nt + j];
             w.sg[i * kMaxTypes + j] = (float)std::sqrt(d->dissipative[i * nt + j]);
         }
+    if (d->cell_conservative || d->cell_dissipative) {
+        // per-cell strengths, rounded to FP32 (the pair's mean is taken in FP32)
+        if (!d->cell_conservative || !d->cell_dissipative)
+            return fail(PDPD_ERR_INVALID, "cell_conservative and cell_dissipative must be given together");
+        const size_t m = (size_t)w.n_cells() * nt * nt;
+        w.cell_par.resize(m);
+        for (size_t k = 0; k < m; k++) {
+            const double a = d->cell_conservative[k], g = d->cell_dissipative[k];
+            if (!std::isfinite(a) || !std::isfinite(g) || g < 0)
+                return fail(PDPD_ERR_INVALID, "per-cell strengths must be finite, dissipative strengths non-negative (cell " +
+                                                  std::to_string(k / ((size_t)nt * nt)) + ")");
+            w.cell_par[k] = make_float2((float)a, (float)g);
+        }
+    }
     // import: FP64 AoS -> FP32, w = v - f dt/2 (dpd_engine_avx2_half_merge_tbb.hpp:515-524)
     const float dt = (float)d->dt, hdt = 0.5f * dt;
     w.up_pos.resize(w.n);

@@ -30,7 +30,8 @@ struct WorldDev {
     uint32_t seed_lo, seed_hi;
     uint32_t n_types;
     uint32_t stationary;   // bit k set: bead type k is stationary (never moves, v = f = 0 after a step)
-    uint32_t pad0[2];
+    uint32_t spatial;      // 1: strengths per cell in cell_par (a and sg below unused)
+    uint32_t cell_par_off; // first entry of this world's per-cell strengths in cell_par
     float a[kMaxTypes * kMaxTypes];    // conservative a_ij
     float sg[kMaxTypes * kMaxTypes];   // sqrt(gamma_ij)
 };
@@ -96,6 +97,10 @@ struct DeviceState {
     TermRec *terms;
     double *term_frc;          // kTermFrcStride per term, written during k_scatter each step
     StationaryBead *stationary_beads;
+    // Spatially varying strengths: per world cell (local cell id) and ordered
+    // type pair (home type * n_types + other type), (a, gamma) in FP32. A pair
+    // uses the mean of its two cells' values.
+    const float2 *cell_par;
 };
 
 }  // namespace pdpd

@@ -21,6 +21,7 @@ struct StepLaunch {
     uint32_t force_variant;  // pair-force kernel variant (loads in flight / occupancy), same results
     uint32_t *tile_sum;      // scan_tiles(n_cells) entries: tile sums of the three-phase scan
     uint32_t lookback_tiles; // single-pass look-back scan up to this many tiles, three-phase above
+    uint32_t spatial;        // some world has per-cell strengths (the pair-force kernel with the cell lookup)
 };
 
 constexpr uint32_t kForceVariants = 4;

@@ -594,7 +594,11 @@ __device__ __forceinline__ void wave_sync()
 // kBondWave: a fourth wave sums the bonded terms beside the pair waves;
 // otherwise the middle plane's wave sums them after its pairs (three waves per
 // workgroup, more workgroups resident). Same sums either way.
-template <int kScanWidth, int kWaves, bool kBondWave>
+// kSpatial: worlds flagged spatial take a pair's strengths as the mean of the
+// values at the two beads' cells (naive_dpd_engine.hpp:452-468): a = (a_h +
+// a_o) / 2, sqrt(gamma) = sqrtf((g_h + g_o) / 2), index home type * n_types +
+// other type; other worlds use the constant tables as without it.
+template <int kScanWidth, int kWaves, bool kBondWave, bool kSpatial>
 __global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
 __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
 {
@@ -652,6 +656,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
             const float scale = W->rng_scale;
             const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
             const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
+            const bool spatial = kSpatial && W->spatial;
             const int cx = cell_coord(hp.x, L0), cy = cell_coord(hp.y, L1), cz = cell_coord(hp.z, L2);
 
             // the plane's three rows; per row a stream of at most two slot ranges,
@@ -729,10 +734,25 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                     const float dvx = v.x - ov.x, dvy = v.y - ov.y, dvz = v.z - ov.z;
                     const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
                     const uint32_t ph = fbits(p.w), oh = fbits(op.w);
-                    const uint32_t ti = (ph >> 28) * kMaxTypes + (oh >> 28);
-                    const float sgw = s_sg[ti] * wr;
+                    float a, sg;
+                    if (kSpatial && spatial) {
+                        const uint32_t nt = W->n_types, ti = (ph >> 28) * nt + (oh >> 28);
+                        const uint32_t hc = (uint32_t)((cell_coord(p.x, L0) * L1 + cell_coord(p.y, L1)) * L2 +
+                                                       cell_coord(p.z, L2));
+                        const uint32_t oc = (uint32_t)((cell_coord(op.x, L0) * L1 + cell_coord(op.y, L1)) * L2 +
+                                                       cell_coord(op.z, L2));
+                        const float2 ph_ = S.cell_par[W->cell_par_off + hc * nt * nt + ti];
+                        const float2 po_ = S.cell_par[W->cell_par_off + oc * nt * nt + ti];
+                        a = (ph_.x + po_.x) * 0.5f;
+                        sg = sqrtf((ph_.y + po_.y) * 0.5f);
+                    } else {
+                        const uint32_t ti = (ph >> 28) * kMaxTypes + (oh >> 28);
+                        a = s_a[ti];
+                        sg = s_sg[ti];
+                    }
+                    const float sgw = sg * wr;
                     const float u = rng_uniform(rng_sym(th_lo, th_hi, ph, oh));
-                    const float F = fmaf(s_a[ti], wr, sgw * fmaf(-sgw, rdotv, scale * u));
+                    const float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, scale * u));
                     qres[i] = make_float4(ex, ey, ez, F);
                 }
                 wave_sync();
@@ -882,8 +902,22 @@ __global__ __launch_bounds__(kBeadThreads) void k_trace_pairs(DeviceState S, int
                     const float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
                     const float dvx = hv.x - ov.x, dvy = hv.y - ov.y, dvz = hv.z - ov.z;
                     const float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                    const uint32_t ti = (hh >> 28) * kMaxTypes + (oh >> 28);
-                    const float a = W->a[ti], sg = W->sg[ti];
+                    float a, sg, gamma;
+                    if (W->spatial) {
+                        const uint32_t nt = W->n_types, ti = (hh >> 28) * nt + (oh >> 28);
+                        const uint32_t hc = (uint32_t)((c[0] * L[1] + c[1]) * L[2] + c[2]);
+                        const uint32_t oc = (uint32_t)((n[0] * L[1] + n[1]) * L[2] + n[2]);
+                        const float2 ph_ = S.cell_par[W->cell_par_off + hc * nt * nt + ti];
+                        const float2 po_ = S.cell_par[W->cell_par_off + oc * nt * nt + ti];
+                        a = (ph_.x + po_.x) * 0.5f;
+                        gamma = (ph_.y + po_.y) * 0.5f;
+                        sg = sqrtf(gamma);
+                    } else {
+                        const uint32_t ti = (hh >> 28) * kMaxTypes + (oh >> 28);
+                        a = W->a[ti];
+                        sg = W->sg[ti];
+                        gamma = sg * sg;
+                    }
                     const float sgw = sg * wr;
                     const float u = rng_uniform(rng_sym(th_lo, th_hi, hh, oh));
                     const float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, scale * u));
@@ -893,7 +927,7 @@ __global__ __launch_bounds__(kBeadThreads) void k_trace_pairs(DeviceState S, int
                     r.dx[0] = dx; r.dx[1] = dy; r.dx[2] = dz;
                     r.dr = rr;
                     r.dv[0] = dvx; r.dv[1] = dvy; r.dv[2] = dvz;
-                    r.diss_strength = sg * sg;
+                    r.diss_strength = gamma;
                     r.inv_root_dt = scale;
                     r.gammap = sgw * sgw;
                     r.rng = u;
@@ -990,13 +1024,16 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
     Sp.ctr = L.S.ctr + L.cur * L.n_worlds;       // counters read by that step
     const int nxt = L.cur ^ 1;
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
-    if (fg) {
-        constexpr uint32_t t3 = kForceBeads * kForcePlanes;
+    constexpr uint32_t t3 = kForceBeads * kForcePlanes;
+    if (fg && L.spatial) {
+        // any world with spatially varying strengths: one shape, with the per-cell lookup
+        k_force<3, 5, false, true><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks);
+    } else if (fg) {
         switch (L.force_variant) {
-        case 1: k_force<4, 4, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        case 2: k_force<2, 5, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        case 3: k_force<3, 5, true><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        default: k_force<3, 5, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 1: k_force<4, 4, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 2: k_force<2, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 3: k_force<3, 5, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        default: k_force<3, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
         }
     }
     return hipGetLastError();

@@ -40,7 +40,7 @@ double now_s()
 // Deep copy of a pdpd_world_desc.
 struct OwnedDesc {
     pdpd_world_desc d{};
-    std::vector<double> cons, diss, x, v, f, bk, br, ak, at0;
+    std::vector<double> cons, diss, x, v, f, bk, br, ak, at0, ccons, cdiss;
     std::vector<uint8_t> stat, btype, mono, poff;
     std::vector<uint32_t> pid, ptbs, bh, bt, ptas, ahb, atb, ptype, pbs, pbi;
 
@@ -78,6 +78,9 @@ struct OwnedDesc {
         d.polymer_type = cp(ptype, s.polymer_type, np);
         d.polymer_bead_start = cp(pbs, s.polymer_bead_start, np + 1);
         d.polymer_bead_ids = cp(pbi, s.polymer_bead_ids, np ? s.polymer_bead_start[np] : 0);
+        const size_t ncp = (size_t)s.box[0] * (size_t)s.box[1] * (size_t)s.box[2] * nt * nt;
+        d.cell_conservative = cp(ccons, s.cell_conservative, ncp);
+        d.cell_dissipative = cp(cdiss, s.cell_dissipative, ncp);
     }
 };
 
@@ -94,12 +97,14 @@ struct Request {
 };
 
 // Device bytes one resident world needs: double-buffered pos/vel/frc (96 B),
-// sort keys, ranks, permutation and bead->slot map (16 B) per bead, and the
-// cell counts and starts (8 B) per cell.
+// sort keys, ranks, permutation and bead->slot map (16 B) per bead, the cell
+// counts and starts (8 B) per cell, and per-cell strengths (8 B per cell and
+// type pair) if it has them.
 uint64_t device_bytes(const pdpd_world_desc &d)
 {
     const uint64_t cells = (uint64_t)d.box[0] * (uint64_t)d.box[1] * (uint64_t)d.box[2];
-    return 112ull * d.n_beads + 8ull * cells + 4096;
+    const uint64_t par = d.cell_conservative ? 8ull * cells * d.n_bead_types * d.n_bead_types : 0;
+    return 112ull * d.n_beads + 8ull * cells + par + 4096;
 }
 
 }  // namespace

@@ -23,6 +23,9 @@ class PdpdError(RuntimeError):
         self.code = code
 
 
+ABI_VERSION = 2          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
+
+
 class WorldDesc(C.Structure):
     _fields_ = [
         ("box", C.c_double * 3), ("dt", C.c_double), ("lambda_", C.c_double),
@@ -38,6 +41,7 @@ class WorldDesc(C.Structure):
         ("angle_kappa", C.c_void_p), ("angle_theta0", C.c_void_p),
         ("n_polymers", C.c_uint32), ("polymer_type", C.c_void_p), ("polymer_bead_start", C.c_void_p),
         ("polymer_bead_ids", C.c_void_p),
+        ("cell_conservative", C.c_void_p), ("cell_dissipative", C.c_void_p),
     ]
 
 
@@ -147,6 +151,8 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.pdpd_abi_version() != ABI_VERSION:
+        raise PdpdError(-3, f"{p}: ABI version {lib.pdpd_abi_version()}, these bindings are for {ABI_VERSION}")
     if path is None:
         _lib = lib
     return lib
@@ -222,6 +228,10 @@ def world_desc(world):
     d.polymer_type = _ptr(keep(world.polymer_type_of, np.uint32))
     d.polymer_bead_start = _ptr(keep(world.polymer_bead_start, np.uint32))
     d.polymer_bead_ids = _ptr(keep(world.polymer_bead_ids, np.uint32))
+    if world.spatial:
+        cc, cd = world.cell_strengths()
+        d.cell_conservative = _ptr(keep(cc.reshape(-1), np.float64))
+        d.cell_dissipative = _ptr(keep(cd.reshape(-1), np.float64))
     return d, keep
 
 

@@ -4,7 +4,8 @@ three formats (dpd_state_io.hpp:109-588): text "v0", and "v1binary" /
 include/pdpd/world_state_io.hpp, whose header comment gives the record layout;
 both write the same bytes, and those are the bytes the reference writes for the
 same state (text numbers in C++'s default 6-significant-digit format, constant
-parameters as std::to_string's %f). Files ending in .gz / .bz2 are compressed.
+parameters as std::to_string's %f, expressions with every binary operation in
+parentheses). Files ending in .gz / .bz2 are compressed.
 Binary records are parsed with one Python step per polymer."""
 from __future__ import annotations
 
@@ -14,6 +15,7 @@ import struct
 
 import numpy as np
 
+from . import expr
 from .world import PolymerType, World
 
 
@@ -140,10 +142,13 @@ def dumps(w: World, precise=False, bead_radius=0.5, binary=False):
            f"T {int(w.t)} {g(w.dt)}", f"Lambda {g(w.lambda_)}",
            "Origin " + " ".join(g(o) for o in origin), "Box " + " ".join(g(b) for b in w.box),
            f"Seed {int(w.seed)}"]
+    ex = w.exprs or {}
     for i in range(nt):
-        out.append(f"ConservativeStrength {i} " + " ".join(_f(w.conservative[i, j]) for j in range(nt)))
+        out.append(f"ConservativeStrength {i} " + " ".join(ex.get(("conservative", i, j)) or _f(w.conservative[i, j])
+                                                           for j in range(nt)))
     for i in range(nt):
-        out.append(f"DissipativeStrength {i} " + " ".join(_f(w.dissipative[i, j]) for j in range(nt)))
+        out.append(f"DissipativeStrength {i} " + " ".join(ex.get(("dissipative", i, j)) or _f(w.dissipative[i, j])
+                                                          for j in range(nt)))
     out += ["", "# BeadTypes"]
     stat = w.stationary if w.stationary is not None else np.zeros(nt, np.uint8)
     for i, name in enumerate(w.bead_type_names):
@@ -202,6 +207,34 @@ def _num(s):
         return float(s)
     except ValueError:
         raise ValueError(f"not a number (parameter expressions are not supported): '{s}'") from None
+
+
+def _strength(tok):
+    """(value, None) for a number or a constant expression, (0.0, text) for
+    an expression of variables, text in the reference's printed form."""
+    try:
+        return float(tok), None
+    except ValueError:
+        pass
+    tree = expr.parse(tok)
+    if tree[0] == "c":
+        return tree[1], None
+    return 0.0, expr.text(tree)
+
+
+def _strengths(L, name, nt, kind, exprs):
+    rows = []
+    for _ in range(nt):
+        p = L.expect(name, nt + 2)
+        i = int(p[1])
+        row = []
+        for j, tok in enumerate(p[2:]):
+            val, text = _strength(tok)
+            if text is not None:
+                exprs[(kind, i, j)] = text
+            row.append(val)
+        rows.append(row)
+    return np.array(rows, dtype=np.float64)
 
 
 def _read_binary_beads(blob, v2, box, ptypes, ptype_of, members, x, v, f):
@@ -302,8 +335,9 @@ def loads(data) -> World:
     origin = np.array([_num(c) for c in L.expect("Origin", 4)[1:]])
     box = np.array([_num(c) for c in L.expect("Box", 4)[1:]])
     seed = int(L.expect("Seed", 2)[1])
-    cons = np.array([[_num(c) for c in L.expect("ConservativeStrength", nt + 2)[2:]] for _ in range(nt)])
-    diss = np.array([[_num(c) for c in L.expect("DissipativeStrength", nt + 2)[2:]] for _ in range(nt)])
+    exprs = {}
+    cons = _strengths(L, "ConservativeStrength", nt, "conservative", exprs)
+    diss = _strengths(L, "DissipativeStrength", nt, "dissipative", exprs)
     names, stat = [], np.zeros(nt, np.uint8)
     for i in range(nt):
         p = L.expect("BeadType")
@@ -377,7 +411,7 @@ def loads(data) -> World:
                  dissipative=diss, polymer_types=ptypes, polymer_type_of=ptype_of.astype(np.uint32),
                  polymer_bead_start=starts, polymer_bead_ids=np.asarray(ids, np.uint32), x=x, v=v, f=f,
                  bead_type=bead_type, is_monomer=mono, polymer_id=polymer_id, polymer_offset=offset,
-                 stationary=stat if stat.any() else None, origin=origin)
+                 stationary=stat if stat.any() else None, origin=origin, exprs=exprs or None)
 
 
 def read(name) -> World:

@@ -55,6 +55,10 @@ class World:
     polymer_offset: np.ndarray     # (n,) uint8
     stationary: np.ndarray = None  # (nt,) uint8 or None
     origin: np.ndarray = None      # (3,) offset of the box corner in files, or None for 0
+    # strengths given as expressions of the cell position (x, y, z, ux, uy, uz):
+    # {("conservative" | "dissipative", i, j): expression text}; those entries of
+    # the constant matrices are unused (0). None or {}: constant strengths.
+    exprs: dict = None
 
     @property
     def n_beads(self):
@@ -63,6 +67,56 @@ class World:
     @property
     def n_types(self):
         return int(self.conservative.shape[0])
+
+    @property
+    def spatial(self):
+        return bool(self.exprs)
+
+    def set_expression(self, kind, i, j, src, symmetric=True):
+        """Make interaction (i, j) (and (j, i)) of kind "conservative" or
+        "dissipative" the expression src; a bare number sets the constant
+        matrix instead (an expression of no variable stays an expression, as
+        in the reference's Parameter). The text is kept in the reference's printed
+        form (every binary operation in parentheses)."""
+        from . import expr
+        if kind not in ("conservative", "dissipative"):
+            raise ValueError(f"unknown strength kind '{kind}'")
+        tree = expr.parse(src)
+        unknown = expr.variables(tree) - set(expr.SPACE)
+        if unknown:
+            raise ValueError(f"variables other than the cell position: {sorted(unknown)}")
+        for a, b in {(i, j), (j, i)} if symmetric else {(i, j)}:
+            mat = self.conservative if kind == "conservative" else self.dissipative
+            if self.exprs is None:
+                self.exprs = {}
+            if tree[0] == "c":
+                mat[a, b] = tree[1]
+                self.exprs.pop((kind, a, b), None)
+            else:
+                mat[a, b] = 0.0
+                self.exprs[(kind, a, b)] = expr.text(tree)
+
+    def cell_strengths(self):
+        """(conservative, dissipative), each (n_cells, nt, nt) float64: the
+        strengths at every cell, cell index (x * Ly + y) * Lz + z, expressions
+        evaluated with the cell's bindings in double precision
+        (naive_dpd_engine.hpp:158-183)."""
+        from . import expr
+        L = [int(b) for b in self.box]
+        nc, nt = L[0] * L[1] * L[2], self.n_types
+        cons = np.broadcast_to(self.conservative, (nc, nt, nt)).copy()
+        diss = np.broadcast_to(self.dissipative, (nc, nt, nt)).copy()
+        trees = {k: expr.parse(v) for k, v in (self.exprs or {}).items()}
+        if not trees:
+            return cons, diss
+        for cx in range(L[0]):
+            for cy in range(L[1]):
+                for cz in range(L[2]):
+                    b = expr.cell_bindings((cx, cy, cz), L)
+                    c = (cx * L[1] + cy) * L[2] + cz
+                    for (kind, i, j), tree in trees.items():
+                        (cons if kind == "conservative" else diss)[c, i, j] = expr.evaluate(tree, b)
+        return cons, diss
 
     @property
     def hash(self):

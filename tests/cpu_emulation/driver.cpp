@@ -9,7 +9,7 @@
 
 struct W {
     double box[3];
-    std::vector<double> cons, diss, x, v, f, bk, br, ak, at;
+    std::vector<double> cons, diss, x, v, f, bk, br, ak, at, ccons, cdiss;
     std::vector<uint8_t> stat, bt, mono, poff;
     std::vector<uint32_t> pid, ptbs, bh, btl, ptas, ahb, atb, ptype, pbs, pbi;
     pdpd_world_desc d{};
@@ -88,11 +88,22 @@ int main(int argc, char **argv)
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
+        {"spatial", {5, 4, 6}, 0.2, true, 0.5, 2, nullptr},      // strengths per cell, batched with a constant world
     };
     for (auto &c : cases) {
         fprintf(stderr, "case %s\n", c.name);
         W w;
         make(w, c.L[0], c.L[1], c.L[2], c.bonded, c.wall, c.th, 7, strcmp(c.name, "dense") ? 3 : 24);
+        if (!strcmp(c.name, "spatial")) {
+            const int nt = (int)w.d.n_bead_types, nc = c.L[0] * c.L[1] * c.L[2];
+            for (int cell = 0; cell < nc; cell++)
+                for (int k = 0; k < nt * nt; k++) {
+                    w.ccons.push_back(20.0 + cell % 7 + k);
+                    w.cdiss.push_back(3.0 + 0.1 * (cell % 5));
+                }
+            w.d.cell_conservative = w.ccons.data();
+            w.d.cell_dissipative = w.cdiss.data();
+        }
         const char *why = pdpd_can_support(&w.d);
         if (why && *why) { fprintf(stderr, "unsupported: %s\n", why); return 1; }
         pdpd_ctx *ctx;
@@ -121,6 +132,13 @@ int main(int argc, char **argv)
             CHECK(pdpd_run(ctx, 1));
             CHECK(pdpd_world_remove(ctx, s2));
             CHECK(pdpd_run(ctx, 1));
+            CHECK(pdpd_sync(ctx));
+        } else if (!strcmp(c.name, "spatial")) {
+            W w2;
+            make(w2, 4, 5, 6, 0.1, false, 0.0, 9);
+            int s2;
+            CHECK(pdpd_world_add(ctx, &w2.d, &s2));
+            CHECK(pdpd_run(ctx, c.steps));
             CHECK(pdpd_sync(ctx));
         } else {
             CHECK(pdpd_run(ctx, c.steps));

@@ -21,8 +21,10 @@
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #define HIP_SYMBOL(x) (&x)
 
+struct float2 { float x, y; };
 struct float3 { float x, y, z; };
 struct float4 { float x, y, z, w; };
+inline float2 make_float2(float a, float b) { return {a, b}; }
 inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 struct dim3_ { uint32_t x, y, z; };

@@ -10,7 +10,8 @@ live work-items, and a wave intrinsic reached by only part of a wave aborts.
   * the plain build's first step equals the FP32 oracle bit for bit (the same
     source as the gfx950 build, FP32 with -ffp-contract=off), including a dense
     world whose accepted pairs overflow the pair kernel's per-bead lists and
-    per-wave queues many times.
+    per-wave queues many times, a world whose strengths vary per cell, and that
+    world batched with a constant-strength one.
 This checks the kernel source on the CPU; GPU parity is tests/test_gpu_*.py."""
 import os
 import subprocess
@@ -55,7 +56,13 @@ worlds = {
     "lattice4": W.make_lattice_world(),
     "dense3": W.make_world((3, 4, 3), density=24.0, seed=16),    # pair lists and queues overflow
 }
-for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2)):
+sp = W.make_world((6, 5, 7), bonded_fraction=0.2, seed=17)
+sp.set_expression("conservative", 0, 0, "25+50*ux")
+sp.set_expression("conservative", 0, 1, "30*uy+x/4")
+sp.set_expression("dissipative", 1, 1, "3+3*uz")
+sp.set_expression("dissipative", 0, 2, "4.5*(1+z/10)")
+worlds["spatial"] = sp                                          # strengths per cell
+for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2, 1)):
     ctx = capi.Context(0)
     ctx.set_config("force_variant", var)
     ctx.set_config("scan_lookback_tiles", (var & 1) * 64)     # both cell-scan forms
@@ -68,6 +75,18 @@ for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2)):
             np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
     print(name, "bitwise" if same else "DIFFERENT", flush=True)
     ctx.close()
+# one context with a spatial and a constant world: each world's own strengths
+ctx = capi.Context(0)
+slots = [ctx.add(worlds["spatial"]), ctx.add(worlds["water4"])]
+ctx.run(2)
+same = True
+for slot, w in zip(slots, (worlds["spatial"], worlds["water4"])):
+    x, ww, f, t = ctx.download_f32(slot)
+    ref = O.F32State(w).step(2)
+    same &= (np.array_equal(x.view(np.uint32), ref.x.view(np.uint32)) and
+             np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
+print("mixed", "bitwise" if same else "DIFFERENT", flush=True)
+ctx.close()
 """
 
 
@@ -77,4 +96,4 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 5, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 7, r.stdout
