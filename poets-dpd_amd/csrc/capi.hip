@@ -919,7 +919,7 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
          "\tcell scan: single pass with look-back up to this many 2048-cell tiles, three phases above\n";
     s += "force_variant=" + std::to_string(ctx->force_variant) +
          "\tpair-force kernel variant: 0 = 3 candidate loads in flight, bonded sums after the middle plane's pairs;"
-         " 1 = 2 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave\n";
+         " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }

@@ -602,6 +602,9 @@ template <int kScanWidth, int kWaves, bool kBondWave, bool kSpatial>
 __global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
 __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
 {
+    // a batch accepts at most 64 * kScanWidth pairs, which an emptied queue must hold
+    static_assert(kQueue >= kForceBeads * kScanWidth && kQueue <= 256, "queue positions are 8-bit");
+    static_assert(kLaneList >= kScanWidth, "a lane lists up to kScanWidth pairs per batch");
     constexpr int kThreads = kBondWave ? kForceThreads : kForceBeads * kForcePlanes;
     constexpr int kPairLanes = kForceBeads * kForcePlanes;
     // per pair wave: queued pairs (candidate slot | image code bits 26..31, and
@@ -1027,7 +1030,7 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
     constexpr uint32_t t3 = kForceBeads * kForcePlanes;
     if (fg && L.spatial) {
         // any world with spatially varying strengths: one shape, with the per-cell lookup
-        k_force<3, 5, false, true><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks);
+        k_force<3, 4, false, true><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks);
     } else if (fg) {
         switch (L.force_variant) {
         case 1: k_force<4, 4, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
