@@ -217,12 +217,17 @@ private:
     pdpd_ctx *m_ctx = nullptr;
 };
 
-inline void download_into(pdpd_ctx *ctx, int slot, WorldState &s)
+// The world's state (after all enqueued work), or with from_snapshot its last
+// pdpd_world_snapshot (later steps may still run), into s.
+inline void download_into(pdpd_ctx *ctx, int slot, WorldState &s, bool from_snapshot = false)
 {
     const size_t n = s.beads.size();
     std::vector<double> x(3 * n), v(3 * n), f(3 * n);
     int64_t t = 0;
-    check(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t), "pdpd_world_download");
+    if (from_snapshot)
+        check(pdpd_world_snapshot_download(ctx, slot, x.data(), v.data(), f.data(), &t), "pdpd_world_snapshot_download");
+    else
+        check(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t), "pdpd_world_download");
     for (size_t i = 0; i < n; i++) {
         auto &b = s.beads[i];
         for (int d = 0; d < 3; d++) { b.x[d] = x[3 * i + d]; b.v[d] = v[3 * i + d]; b.f[d] = f[3 * i + d]; }
@@ -321,8 +326,11 @@ public:
     }
 
     // Interval run with the next interval already on the device while the
-    // callback of the previous one runs on the host (the contract of
-    // dpd_engine.hpp:187-194 allows execution to continue during the callback).
+    // previous one's state comes to the host and its callback runs (the
+    // contract of dpd_engine.hpp:187-194 allows execution to continue during
+    // the callback): a device-side snapshot is enqueued after each interval and
+    // the next interval behind it, so the copy to the host and the callback
+    // overlap the next interval's steps.
     unsigned Run(int interval_count, unsigned interval_size, std::function<bool()> interval_callback) override
     {
         require_attached();
@@ -341,10 +349,11 @@ public:
         }
         pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
         for (;;) {
-            export_state();
+            pdpd_host::check(pdpd_world_snapshot(ctx(), m_slot), "pdpd_world_snapshot");
             done += interval_size;
             const bool more = interval_count < 0 || interval_count > 1;
             if (more) pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
+            export_state(true);
             const bool cont = interval_callback();
             if (interval_count > 0) --interval_count;
             if (more && !m_snap.same(*m_state)) {
@@ -390,9 +399,9 @@ protected:
         import_state();
     }
 
-    void export_state()
+    void export_state(bool from_snapshot = false)
     {
-        pdpd_host::download_into(ctx(), m_slot, *m_state);
+        pdpd_host::download_into(ctx(), m_slot, *m_state, from_snapshot);
         m_snap.take(*m_state);
     }
 

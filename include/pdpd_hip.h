@@ -142,6 +142,18 @@ int pdpd_sync(pdpd_ctx *ctx);
  * Any output pointer may be NULL. */
 int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t);
 
+/* Interval snapshots overlapped with compute (the Run(intervals) contract,
+ * dpd_engine.hpp:187-215, with run_world.cpp:177-275's callback per interval).
+ * pdpd_world_snapshot enqueues a device-side copy of one world's state as it
+ * will be when the work enqueued so far has run, and returns at once; more
+ * steps may be enqueued behind it. pdpd_world_snapshot_download waits for that
+ * copy only (not for later steps), brings it to the host on a second stream
+ * and exports it as pdpd_world_download does, with the t of the snapshot. One
+ * snapshot per context is held; a new one replaces it. A bead that had left
+ * the box by the snapshot is reported here as PDPD_ERR_STATE. */
+int pdpd_world_snapshot(pdpd_ctx *ctx, int slot);
+int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t);
+
 /* Raw FP32 engine state of one world in bead-index order: x, the shifted
  * velocity w = v - f*dt/2, f; and the device step counter. For parity tests. */
 int pdpd_world_download_f32(pdpd_ctx *ctx, int slot, float *x, float *w, float *f, uint32_t *t);

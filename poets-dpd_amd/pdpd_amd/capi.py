@@ -109,6 +109,9 @@ SIGNATURES = {
     "pdpd_sync": (C.c_int, [C.c_void_p]),
     "pdpd_world_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_int64)]),
+    "pdpd_world_snapshot": (C.c_int, [C.c_void_p, C.c_int]),
+    "pdpd_world_snapshot_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.POINTER(C.c_int64)]),
     "pdpd_world_download_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.POINTER(C.c_uint32)]),
     "pdpd_world_download_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -312,6 +315,18 @@ class Context:
         x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3))
         t = C.c_int64()
         check(self.lib.pdpd_world_download(self.h, slot, _ptr(x), _ptr(v), _ptr(f), C.byref(t)))
+        return x, v, f, t.value
+
+    def snapshot(self, slot):
+        """Enqueue a copy of the world's state behind the work enqueued so far."""
+        check(self.lib.pdpd_world_snapshot(self.h, slot))
+
+    def snapshot_download(self, slot):
+        """The last snapshot of the slot as (x, v, f, t), without waiting for later steps."""
+        n = self.worlds[slot][0]
+        x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3))
+        t = C.c_int64()
+        check(self.lib.pdpd_world_snapshot_download(self.h, slot, _ptr(x), _ptr(v), _ptr(f), C.byref(t)))
         return x, v, f, t.value
 
     def download_f32(self, slot):

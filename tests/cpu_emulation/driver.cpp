@@ -156,6 +156,15 @@ int main(int argc, char **argv)
         std::vector<double> x(3 * w.d.n_beads), v(x.size()), f(x.size());
         int64_t t;
         CHECK(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t));
+        {   // a snapshot taken before one more step holds the state before it
+            std::vector<double> sx(x.size()), sv(x.size()), sf(x.size());
+            int64_t st;
+            CHECK(pdpd_world_snapshot(ctx, slot));
+            CHECK(pdpd_run(ctx, 1));
+            CHECK(pdpd_world_snapshot_download(ctx, slot, sx.data(), sv.data(), sf.data(), &st));
+            if (st != t || sx != x || sv != v || sf != f) { fprintf(stderr, "snapshot differs\n"); return 1; }
+            CHECK(pdpd_world_download(ctx, slot, x.data(), v.data(), f.data(), &t));
+        }
         double fs = 0;
         for (double q : f) fs += std::fabs(q);
         fprintf(stderr, "  ok t=%lld sum|f|=%g\n", (long long)t, fs);

@@ -211,6 +211,12 @@ inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.001f; return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = malloc(n); return *p ? hipSuccess : 2; }
+inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
+#define hipEventDisableTiming 2u
+#define hipHostMallocDefault 0u
 struct EmuGraph { std::vector<std::function<void()>> ops; };
 inline hipError_t hipStreamBeginCapture(hipStream_t, int) { emu::st().capturing = true; emu::st().captured.clear(); return hipSuccess; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {

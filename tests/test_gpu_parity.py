@@ -332,3 +332,33 @@ def test_force_timing_hook_leaves_state_unchanged(gpu_lib):
         assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
     a.close()
     b.close()
+
+
+def test_interval_snapshot_overlaps_later_steps(gpu_lib):
+    """pdpd_world_snapshot: the state at the snapshot, bit for bit, while the
+    steps enqueued behind it run; the state after them is unchanged by it."""
+    w = W.make_world((10, 9, 8), bonded_fraction=0.2, seed=61)
+    _say("snapshot after 5 steps, 7 more enqueued behind it")
+    ctx = capi.Context(0)
+    slot = ctx.add(w)
+    ctx.run(5)
+    ctx.snapshot(slot)
+    ctx.run(7)
+    xs, vs, fs, ts = ctx.snapshot_download(slot)
+    x12, v12, f12, t12 = ctx.download(slot)
+    ctx.close()
+    ref = capi.Context(0)
+    rs = ref.add(w)
+    ref.run(5)
+    x5, v5, f5, t5 = ref.download(rs)
+    ref.run(7)
+    xr, vr, fr, tr = ref.download(rs)
+    ref.close()
+    assert ts == t5 == 5 and t12 == tr == 12
+    for a, b in ((xs, x5), (vs, v5), (fs, f5), (x12, xr), (v12, vr), (f12, fr)):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    c = capi.Context(0)
+    s1 = c.add(w)
+    with pytest.raises(capi.PdpdError, match="no snapshot"):
+        c.snapshot_download(s1)
+    c.close()
