@@ -218,7 +218,7 @@ private:
 };
 
 // The world's state (after all enqueued work), or with from_snapshot its last
-// pdpd_world_snapshot (later steps may still run), into s.
+// pdpd_snapshot (later steps may still run), into s.
 inline void download_into(pdpd_ctx *ctx, int slot, WorldState &s, bool from_snapshot = false)
 {
     const size_t n = s.beads.size();
@@ -349,7 +349,7 @@ public:
         }
         pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");
         for (;;) {
-            pdpd_host::check(pdpd_world_snapshot(ctx(), m_slot), "pdpd_world_snapshot");
+            pdpd_host::check(pdpd_snapshot(ctx()), "pdpd_snapshot");
             done += interval_size;
             const bool more = interval_count < 0 || interval_count > 1;
             if (more) pdpd_host::check(pdpd_run(ctx(), interval_size), "pdpd_run");

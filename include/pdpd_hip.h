@@ -144,14 +144,17 @@ int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f
 
 /* Interval snapshots overlapped with compute (the Run(intervals) contract,
  * dpd_engine.hpp:187-215, with run_world.cpp:177-275's callback per interval).
- * pdpd_world_snapshot enqueues a device-side copy of one world's state as it
- * will be when the work enqueued so far has run, and returns at once; more
+ * pdpd_snapshot enqueues a device-side copy of every resident world's state as
+ * it will be when the work enqueued so far has run, and returns at once; more
  * steps may be enqueued behind it. pdpd_world_snapshot_download waits for that
- * copy only (not for later steps), brings it to the host on a second stream
- * and exports it as pdpd_world_download does, with the t of the snapshot. One
- * snapshot per context is held; a new one replaces it. A bead that had left
- * the box by the snapshot is reported here as PDPD_ERR_STATE. */
-int pdpd_world_snapshot(pdpd_ctx *ctx, int slot);
+ * copy only (not for later steps), brings one world's share to the host on a
+ * second stream and exports it as pdpd_world_download does, with the world's
+ * t at the snapshot; worlds removed since remain downloadable, worlds added
+ * since the last pdpd_run are not in it. One snapshot
+ * is held per context (48 bytes of device memory per resident bead); a new
+ * one replaces it. A bead that had left the box by the snapshot is reported
+ * here as PDPD_ERR_STATE. */
+int pdpd_snapshot(pdpd_ctx *ctx);
 int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t);
 
 /* Raw FP32 engine state of one world in bead-index order: x, the shifted
@@ -273,7 +276,7 @@ typedef struct pdpd_request_result {
 
 typedef struct pdpd_server_stats {
     uint64_t requests, completed, intervals, max_resident;
-    double makespan_s, gpu_busy_s;
+    double makespan_s, gpu_busy_s;                  /* gpu_busy_s: time with a request resident */
     double bead_steps, bead_steps_per_s;             /* sum of beads x steps over the makespan */
     double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
 } pdpd_server_stats;

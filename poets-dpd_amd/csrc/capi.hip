@@ -149,19 +149,22 @@ struct pdpd_ctx {
     DevBuf<float2> cell_par;
     bool spatial = false;                 // some resident world has per-cell strengths
 
-    // interval snapshot: one world's pos, vel, frc (and the flags word) copied
-    // on the device behind the enqueued steps, then to pinned host memory on
-    // a second stream
+    // interval snapshot: pos, vel, frc of every resident world (and the flags
+    // word) copied on the device behind the enqueued steps; a world's share
+    // goes to pinned host memory on a second stream when it is downloaded
+    struct SnapWorld {
+        uint32_t bead_off, n;
+        int64_t t;
+        double dt;
+    };
     hipStream_t copy_stream = nullptr;
     hipEvent_t snap_event = nullptr;
-    DevBuf<float4> snap;
+    DevBuf<float4> snap;             // [pos | vel | frc], snap_beads each
     DevBuf<uint32_t> snap_flags;
+    uint32_t snap_beads = 0;
+    std::map<int, SnapWorld> snap_worlds;
     float4 *snap_host = nullptr;
     size_t snap_host_n = 0;
-    int snap_slot = -1;
-    uint32_t snap_n = 0, snap_bead_off = 0;
-    int64_t snap_t = 0;
-    double snap_dt = 0;
     int cur = 0;
     uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0, n_terms = 0;
     std::vector<int> order;               // slots in layout order
@@ -824,16 +827,15 @@ int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f
     return PDPD_OK;
 }
 
-int pdpd_world_snapshot(pdpd_ctx *ctx, int slot)
+int pdpd_snapshot(pdpd_ctx *ctx)
 {
-    WorldHost *w = ctx ? find_world(ctx, slot) : nullptr;
-    if (!w) return fail(PDPD_ERR_INVALID, "unknown slot");
+    if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
     HIP_TRY(hipSetDevice(ctx->device));
-    int rc = ensure_layout(ctx);
-    if (rc) return rc;
+    // the worlds of the current device layout; worlds added since (not yet on
+    // the device) are not in it, and no relayout is forced here
     if (!ctx->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
     if (!ctx->snap_event) HIP_TRY(hipEventCreateWithFlags(&ctx->snap_event, hipEventDisableTiming));
-    const size_t n = w->n;
+    const size_t n = ctx->n_beads;
     if (ctx->snap.n < 3 * n || !ctx->snap.p) {
         // the stream may still read the old buffer: let it finish first
         if (ctx->snap.p) HIP_TRY(hipEventSynchronize(ctx->snap_event));
@@ -842,32 +844,33 @@ int pdpd_world_snapshot(pdpd_ctx *ctx, int slot)
     if (!ctx->snap_flags.p) HIP_TRY(dev_alloc(ctx->snap_flags, 1));
     const int c = ctx->cur;        // the buffer the enqueued steps end in
     if (n) {
-        HIP_TRY(hipMemcpyAsync(ctx->snap.p, ctx->pos[c].p + w->bead_off, n * sizeof(float4), hipMemcpyDeviceToDevice,
+        HIP_TRY(hipMemcpyAsync(ctx->snap.p, ctx->pos[c].p, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->snap.p + n, ctx->vel[c].p, n * sizeof(float4), hipMemcpyDeviceToDevice,
                                ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->snap.p + n, ctx->vel[c].p + w->bead_off, n * sizeof(float4),
-                               hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->snap.p + 2 * n, ctx->frc[c].p + w->bead_off, n * sizeof(float4),
-                               hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->snap.p + 2 * n, ctx->frc[c].p, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                               ctx->stream));
     }
     if (ctx->flags.p)
         HIP_TRY(hipMemcpyAsync(ctx->snap_flags.p, ctx->flags.p, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     else
         HIP_TRY(launch_fill_u32(ctx->snap_flags.p, 1, 0, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->snap_event, ctx->stream));
-    ctx->snap_slot = slot;
-    ctx->snap_n = (uint32_t)n;
-    ctx->snap_bead_off = w->bead_off;
-    ctx->snap_t = w->t;
-    ctx->snap_dt = w->dt;
+    ctx->snap_beads = (uint32_t)n;
+    ctx->snap_worlds.clear();
+    for (const auto &kv : ctx->worlds)
+        if (kv.second.resident)
+            ctx->snap_worlds[kv.first] = {kv.second.bead_off, kv.second.n, kv.second.t, kv.second.dt};
     return PDPD_OK;
 }
 
 int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t)
 {
     if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
-    if (ctx->snap_slot < 0 || ctx->snap_slot != slot) return fail(PDPD_ERR_INVALID, "no snapshot of this slot");
+    auto it = ctx->snap_worlds.find(slot);
+    if (it == ctx->snap_worlds.end()) return fail(PDPD_ERR_INVALID, "no snapshot of this slot");
+    const pdpd_ctx::SnapWorld sw = it->second;
     HIP_TRY(hipSetDevice(ctx->device));
-    const size_t n = ctx->snap_n;
+    const size_t n = sw.n, N = ctx->snap_beads;
     if (ctx->snap_host_n < 3 * n + 1) {
         if (ctx->snap_host) HIP_TRY(hipHostFree(ctx->snap_host));
         ctx->snap_host = nullptr;
@@ -876,9 +879,9 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
         ctx->snap_host_n = 3 * n + 1;
     }
     HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->snap_event, 0));
-    if (n)
-        HIP_TRY(hipMemcpyAsync(ctx->snap_host, ctx->snap.p, 3 * n * sizeof(float4), hipMemcpyDeviceToHost,
-                               ctx->copy_stream));
+    for (int a = 0; a < 3 && n; a++)
+        HIP_TRY(hipMemcpyAsync(ctx->snap_host + a * n, ctx->snap.p + a * N + sw.bead_off, n * sizeof(float4),
+                               hipMemcpyDeviceToHost, ctx->copy_stream));
     HIP_TRY(hipMemcpyAsync(ctx->snap_host + 3 * n, ctx->snap_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->copy_stream));
     HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
@@ -887,9 +890,9 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
     if (flags & PDPD_FLAG_BAD_POSITION)
         return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
     const float4 *P = ctx->snap_host, *V = ctx->snap_host + n, *F = ctx->snap_host + 2 * n;
-    const double hdt = (double)(float)ctx->snap_dt * 0.5;
+    const double hdt = (double)(float)sw.dt * 0.5;
     for (uint32_t s = 0; s < n; s++) {
-        const uint32_t i = f2u(V[s].w) - ctx->snap_bead_off;
+        const uint32_t i = f2u(V[s].w) - sw.bead_off;
         if (i >= n) return fail(PDPD_ERR_STATE, "corrupt bead index in device state");
         const double px[3] = {P[s].x, P[s].y, P[s].z};
         const double fx[3] = {F[s].x, F[s].y, F[s].z};
@@ -900,7 +903,7 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
             if (v) v[3 * i + k] = wx[k] + fx[k] * hdt;
         }
     }
-    if (t) *t = ctx->snap_t;
+    if (t) *t = sw.t;
     return PDPD_OK;
 }
 

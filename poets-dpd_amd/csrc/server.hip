@@ -90,6 +90,7 @@ struct Request {
     double arrival;         // seconds after the start of the run (scaled)
     uint32_t steps;
     uint32_t done = 0;
+    uint32_t inflight = 0;  // steps enqueued whose results are not yet processed
     int slot = -1;
     uint64_t bytes = 0;
     pdpd_request_result res{};
@@ -198,9 +199,55 @@ int pdpd_server_run(pdpd_server *s)
     const double t0 = now_s();
     uint64_t used = 0;
     for (Request *r : s->resident) used += r->bytes;
-    double busy = 0;
+    double idle = 0;      // time with no request resident
     auto rel = [&]() { return now_s() - t0; };
     auto fail_rc = [&](int rc) { return sfail(rc, pdpd_last_error()); };
+
+    // One interval is computed while the previous one's results are handled:
+    // after an interval is enqueued, the previous interval's snapshot is
+    // downloaded (finished and callback requests) and its callbacks run, then
+    // a snapshot of the new interval is enqueued behind it. Requests finishing
+    // with an interval are frozen (step budget 0) in the next; a request a
+    // callback cancels computes one interval more, which is discarded.
+    std::vector<std::pair<Request *, uint32_t>> pending;    // the snapshotted interval: request, steps advanced
+    bool have_pending = false;
+    auto process = [&]() -> int {
+        for (auto &pa : pending) {
+            Request *r = pa.first;
+            const uint32_t adv = pa.second;
+            r->done += adv;
+            r->inflight -= adv;
+            s->stats.bead_steps += (double)r->res.n_beads * adv;
+            const bool finished = r->done >= r->steps;
+            bool cancel = false;
+            if (s->cb || finished) {
+                const size_t n = r->res.n_beads;
+                r->x.resize(3 * n); r->v.resize(3 * n); r->f.resize(3 * n);
+                int64_t t = 0;
+                int rc = pdpd_world_snapshot_download(s->ctx, r->slot, r->x.data(), r->v.data(), r->f.data(), &t);
+                if (rc) return rc;
+                if (s->cb) cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
+                                          r->f.data()) == 0;
+            }
+            if (finished || cancel) {
+                int rc = pdpd_world_remove(s->ctx, r->slot);
+                if (rc) return rc;
+                r->slot = -1;
+                r->inflight = 0;
+                r->res.done_s = rel();
+                r->res.latency_s = r->res.done_s - r->arrival * scale;
+                r->res.status = finished ? 0 : 3;   // done / cancelled
+                used -= r->bytes;
+                s->stats.completed++;
+            }
+        }
+        std::vector<Request *> keep;
+        for (Request *r : s->resident)
+            if (r->slot >= 0) keep.push_back(r);
+        s->resident.swap(keep);
+        pending.clear();
+        return PDPD_OK;
+    };
 
     while (!s->waiting.empty() || !s->resident.empty()) {
         // admit in arrival order while the head has arrived and fits
@@ -221,56 +268,46 @@ int pdpd_server_run(pdpd_server *s)
         if (s->resident.empty()) {
             const double next = s->waiting.front()->arrival * scale;
             const double wait = next - rel();
-            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            if (wait > 0) {
+                std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+                idle += std::min(wait, 0.05);
+            }
             continue;
         }
         s->stats.max_resident = std::max<uint64_t>(s->stats.max_resident, s->resident.size());
         uint32_t k = 0;
-        for (Request *r : s->resident) k = std::max(k, std::min(K, r->steps - r->done));
-        for (Request *r : s->resident) {
-            int rc = pdpd_world_set_steps(s->ctx, r->slot, std::min(k, r->steps - r->done));
+        for (Request *r : s->resident) k = std::max(k, std::min(K, r->steps - r->done - r->inflight));
+        std::vector<std::pair<Request *, uint32_t>> next;
+        if (k) {
+            for (Request *r : s->resident) {
+                const uint32_t adv = std::min(k, r->steps - r->done - r->inflight);
+                int rc = pdpd_world_set_steps(s->ctx, r->slot, adv);
+                if (rc) return fail_rc(rc);
+                if (adv) next.push_back({r, adv});
+            }
+            int rc = pdpd_run(s->ctx, k);
             if (rc) return fail_rc(rc);
+            for (auto &pa : next) pa.first->inflight += pa.second;
+            s->stats.intervals++;
         }
-        const double tb = now_s();
-        int rc = pdpd_run(s->ctx, k);
-        if (rc) return fail_rc(rc);
-        rc = pdpd_sync(s->ctx);
-        if (rc) return fail_rc(rc);
-        busy += now_s() - tb;
-        s->stats.intervals++;
-        std::vector<Request *> keep;
-        for (Request *r : s->resident) {
-            const uint32_t adv = std::min(k, r->steps - r->done);
-            r->done += adv;
-            s->stats.bead_steps += (double)r->res.n_beads * adv;
-            const bool finished = r->done >= r->steps;
-            bool cancel = false;
-            if (s->cb || finished) {
-                const size_t n = r->res.n_beads;
-                r->x.resize(3 * n); r->v.resize(3 * n); r->f.resize(3 * n);
-                int64_t t = 0;
-                rc = pdpd_world_download(s->ctx, r->slot, r->x.data(), r->v.data(), r->f.data(), &t);
-                if (rc) return fail_rc(rc);
-                if (s->cb) cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
-                                          r->f.data()) == 0;
-            }
-            if (finished || cancel) {
-                rc = pdpd_world_remove(s->ctx, r->slot);
-                if (rc) return fail_rc(rc);
-                r->slot = -1;
-                r->res.done_s = rel();
-                r->res.latency_s = r->res.done_s - r->arrival * scale;
-                r->res.status = finished ? 0 : 3;   // done / cancelled
-                used -= r->bytes;
-                s->stats.completed++;
-            } else {
-                keep.push_back(r);
-            }
+        if (have_pending) {
+            int rc = process();
+            if (rc) return fail_rc(rc);
+            have_pending = false;
         }
-        s->resident.swap(keep);
+        if (k) {
+            for (auto &pa : next)
+                if (pa.first->slot >= 0) pending.push_back(pa);
+            // cancelled requests left the context; their interval is discarded
+            int rc = pdpd_snapshot(s->ctx);
+            if (rc) return fail_rc(rc);
+            have_pending = !pending.empty();
+        }
     }
+    int rc = pdpd_sync(s->ctx);
+    if (rc) return fail_rc(rc);
     s->stats.makespan_s = rel();
-    s->stats.gpu_busy_s += busy;
+    s->stats.gpu_busy_s += s->stats.makespan_s - idle;
     s->stats.requests = s->reqs.size();
     std::vector<double> lat;
     for (auto &kv : s->reqs)

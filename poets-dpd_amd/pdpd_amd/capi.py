@@ -109,7 +109,7 @@ SIGNATURES = {
     "pdpd_sync": (C.c_int, [C.c_void_p]),
     "pdpd_world_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_int64)]),
-    "pdpd_world_snapshot": (C.c_int, [C.c_void_p, C.c_int]),
+    "pdpd_snapshot": (C.c_int, [C.c_void_p]),
     "pdpd_world_snapshot_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.POINTER(C.c_int64)]),
     "pdpd_world_download_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -317,9 +317,9 @@ class Context:
         check(self.lib.pdpd_world_download(self.h, slot, _ptr(x), _ptr(v), _ptr(f), C.byref(t)))
         return x, v, f, t.value
 
-    def snapshot(self, slot):
-        """Enqueue a copy of the world's state behind the work enqueued so far."""
-        check(self.lib.pdpd_world_snapshot(self.h, slot))
+    def snapshot(self):
+        """Enqueue a copy of every resident world's state behind the work enqueued so far."""
+        check(self.lib.pdpd_snapshot(self.h))
 
     def snapshot_download(self, slot):
         """The last snapshot of the slot as (x, v, f, t), without waiting for later steps."""

@@ -159,7 +159,7 @@ int main(int argc, char **argv)
         {   // a snapshot taken before one more step holds the state before it
             std::vector<double> sx(x.size()), sv(x.size()), sf(x.size());
             int64_t st;
-            CHECK(pdpd_world_snapshot(ctx, slot));
+            CHECK(pdpd_snapshot(ctx));
             CHECK(pdpd_run(ctx, 1));
             CHECK(pdpd_world_snapshot_download(ctx, slot, sx.data(), sv.data(), sf.data(), &st));
             if (st != t || sx != x || sv != v || sf != f) { fprintf(stderr, "snapshot differs\n"); return 1; }

@@ -335,14 +335,14 @@ def test_force_timing_hook_leaves_state_unchanged(gpu_lib):
 
 
 def test_interval_snapshot_overlaps_later_steps(gpu_lib):
-    """pdpd_world_snapshot: the state at the snapshot, bit for bit, while the
+    """pdpd_snapshot: the state at the snapshot, bit for bit, while the
     steps enqueued behind it run; the state after them is unchanged by it."""
     w = W.make_world((10, 9, 8), bonded_fraction=0.2, seed=61)
     _say("snapshot after 5 steps, 7 more enqueued behind it")
     ctx = capi.Context(0)
     slot = ctx.add(w)
     ctx.run(5)
-    ctx.snapshot(slot)
+    ctx.snapshot()
     ctx.run(7)
     xs, vs, fs, ts = ctx.snapshot_download(slot)
     x12, v12, f12, t12 = ctx.download(slot)

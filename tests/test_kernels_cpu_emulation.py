@@ -97,3 +97,45 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
     assert sum(l.endswith("bitwise") for l in lines) == 7, r.stdout
+
+
+SERVE_PROBE = r"""
+import sys, numpy as np
+sys.path[:0] = [%(pkg)r]
+from pdpd_amd import capi, world as W
+capi.LIB_PATH = %(lib)r
+rng = np.random.default_rng(5)
+reqs = [(W.make_world((4, 4, 5), bonded_fraction=0.2, seed=600 + i), int(s)) for i, s in enumerate((23, 40, 17, 31))]
+def solo(w, steps):
+    c = capi.Context(0); s = c.add(w); c.run(steps); out = c.download(s); c.close(); return out
+budget = sum(112 * w.n_beads + 8 * int(np.prod(w.box)) + 4096 for w, _ in reqs) // 2
+srv = capi.Server(device=0, hbm_budget_bytes=budget, interval_steps=10, time_scale=0.0)
+seen = {}
+def cb(rid, done, t, x, v, f):
+    seen.setdefault(rid, []).append((done, t))
+    return not (rid == ids[3] and done >= 20)            # the last request is cancelled after 20 steps
+srv.set_callback(cb)
+ids = [srv.submit(w, 0.0, steps) for w, steps in reqs]
+srv.run()
+st = srv.stats()
+ok = st["completed"] == 4 and 1 <= st["max_resident"] < 4
+for rid, (w, steps) in zip(ids[:3], reqs[:3]):
+    r = srv.result(rid)
+    x, v, f, t = solo(w, steps)
+    ok &= r["status"] == 0 and np.array_equal(r["x"], x) and np.array_equal(r["v"], v) and np.array_equal(r["f"], f)
+    dones = [d for d, _ in seen[rid]]
+    ok &= dones == sorted(set(dones)) and dones[-1] == steps and all(d == t for d, t in seen[rid])
+ok &= srv.result(ids[3])["status"] == 3 and [d for d, _ in seen[ids[3]]] == [10, 20]
+print("serving", "ok" if ok else "WRONG", st, flush=True)
+srv.close()
+"""
+
+
+def test_emulated_serving_pipeline(emu_build):
+    """The serving loop with one interval in flight (snapshot, next interval,
+    then downloads and callbacks): queueing under a memory budget, a callback
+    per interval, a cancellation, and results equal to solo runs bit for bit."""
+    code = SERVE_PROBE % dict(pkg=os.path.join(ROOT, "poets-dpd_amd"), lib=os.path.join(emu_build, "libpdpd_emu.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "serving ok" in r.stdout, r.stdout
