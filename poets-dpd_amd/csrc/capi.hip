@@ -57,6 +57,10 @@ double now_s()
 }
 
 constexpr uint32_t kMaxBox = 1024;
+// Chained steps (the pair-force kernel integrating the next step) measured
+// faster at 98k beads (46.1 against 47.5 us per step) and slower at 6.3M
+// (1.77 against 1.74 ms): on by default up to this many resident beads.
+constexpr uint32_t kFuseAutoBeads = 1u << 20;
 constexpr uint32_t kMaxPolymerLength = 127;
 
 uint32_t bead_hash(uint32_t type, bool monomer, uint32_t polymer_id, uint32_t offset)
@@ -136,7 +140,7 @@ struct pdpd_ctx {
 
     // device layout
     DevBuf<float4> pos[2], vel[2], frc[2];
-    DevBuf<uint32_t> key, rank, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
+    DevBuf<uint32_t> key, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
     DevBuf<unsigned long long> scan_tiles;
     DevBuf<uint32_t> tile_sum;
     DevBuf<WorldDev> wdev;
@@ -148,6 +152,7 @@ struct pdpd_ctx {
     DevBuf<StationaryBead> stationary;
     DevBuf<float2> cell_par;
     bool spatial = false;                 // some resident world has per-cell strengths
+    DevBuf<NextBuffers> next;             // [buffer * 2 + counter copy]: what a chained step's force kernel writes
 
     // interval snapshot: pos, vel, frc of every resident world (and the flags
     // word) copied on the device behind the enqueued steps; a world's share
@@ -165,7 +170,9 @@ struct pdpd_ctx {
     std::map<int, SnapWorld> snap_worlds;
     float4 *snap_host = nullptr;
     size_t snap_host_n = 0;
-    int cur = 0;
+    int cur = 0;                          // buffer holding the state after the enqueued steps
+    int cpar = 0;                         // counter copy the next step reads
+    uint32_t all_active = 0;              // the run's first steps in which every world steps (chains stay inside)
     uint32_t n_beads = 0, n_cells = 0, n_worlds = 0, n_bblocks = 0, n_fblocks = 0, n_stationary = 0, n_terms = 0;
     std::vector<int> order;               // slots in layout order
     std::vector<WorldCtr> ctr_host;
@@ -175,11 +182,14 @@ struct pdpd_ctx {
     uint32_t force_grid = 0;
     uint32_t force_variant = 0;
     uint32_t lookback_tiles = 64;
+    uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
+
+    bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_version = ~0ull;
-    int graph_cur = -1;
+    int graph_cur = -1, graph_cpar = -1;
     uint32_t graph_len = 0;
 
     pdpd_stats stats{};
@@ -188,12 +198,13 @@ struct pdpd_ctx {
     {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
-        key.release(); rank.release(); perm.release(); sorted_idx.release(); cell_count.release();
+        key.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
         cell_par.release();
+        next.release();
         snap.release(); snap_flags.release();
         if (snap_host) (void)hipHostFree(snap_host);
         if (snap_event) (void)hipEventDestroy(snap_event);
@@ -205,7 +216,7 @@ struct pdpd_ctx {
     {
         DeviceState S{};
         for (int b = 0; b < 2; b++) { S.pos[b] = pos[b].p; S.vel[b] = vel[b].p; S.frc[b] = frc[b].p; }
-        S.key = key.p; S.rank = rank.p; S.perm = perm.p; S.sorted_idx = sorted_idx.p;
+        S.key = key.p; S.perm = perm.p; S.sorted_idx = sorted_idx.p;
         S.cell_count = cell_count.p; S.cell_start = cell_start.p;
         S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
         S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
@@ -216,11 +227,15 @@ struct pdpd_ctx {
         return S;
     }
 
-    StepLaunch step_launch(int c) const
+    StepLaunch step_launch(int c, int cp, bool integrate, bool fuse) const
     {
         StepLaunch L{};
         L.S = state();
         L.cur = c;
+        L.cpar = cp;
+        L.integrate = integrate ? 1 : 0;
+        L.fuse = fuse ? 1 : 0;
+        L.next = next.p;
         L.n_beads = n_beads; L.n_cells = n_cells; L.n_worlds = n_worlds;
         L.n_bead_blocks = n_bblocks; L.n_force_blocks = n_fblocks;
         L.n_stationary = n_stationary;
@@ -315,6 +330,7 @@ int relayout(pdpd_ctx *ctx)
     ctx->slot_of_bead.release();
     ctx->slot_of_bead = nsob;
     ctx->cur = 0;
+    ctx->cpar = 0;
     std::vector<const WorldHost *> ws;
     for (size_t i = 0; i < order.size(); i++) {
         WorldHost &w = ctx->worlds.at(order[i]);
@@ -334,7 +350,6 @@ int relayout(pdpd_ctx *ctx)
     ctx->n_worlds = (uint32_t)order.size();
 
     HIP_TRY(dev_alloc(ctx->key, nb));
-    HIP_TRY(dev_alloc(ctx->rank, nb));
     HIP_TRY(dev_alloc(ctx->perm, nb));
     HIP_TRY(dev_alloc(ctx->sorted_idx, nb));
     HIP_TRY(dev_alloc(ctx->cell_count, nc));
@@ -426,6 +441,18 @@ int relayout(pdpd_ctx *ctx)
     ctx->n_stationary = (uint32_t)sb.size();
     HIP_TRY(dev_upload(ctx->stationary, sb));
 
+    std::vector<NextBuffers> nbuf(4);
+    for (int c = 0; c < 2; c++)
+        for (int cp = 0; cp < 2; cp++) {
+            NextBuffers &b = nbuf[c * 2 + cp];
+            b.pos = ctx->pos[c].p; b.vel = ctx->vel[c].p; b.frc = ctx->frc[c].p;
+            b.key = ctx->key.p; b.cell_count = ctx->cell_count.p; b.flags = ctx->flags.p;
+            b.scan_tiles = ctx->scan_tiles.p; b.scan_ticket = ctx->scan_ticket.p;
+            b.ctr = ctx->ctr.p + (cp ^ 1) * ctx->n_worlds;
+            b.n_tiles = scan_tiles((uint32_t)nc);
+        }
+    HIP_TRY(dev_upload(ctx->next, nbuf));
+
     ctx->stats.n_worlds = ctx->n_worlds;
     ctx->stats.n_beads = ctx->n_beads;
     ctx->stats.n_force_blocks = ctx->n_fblocks;
@@ -447,15 +474,17 @@ int load_counters(pdpd_ctx *ctx, uint32_t nsteps, uint64_t *bead_steps)
 {
     const uint32_t W = ctx->n_worlds;
     uint64_t bs = 0;
+    ctx->all_active = nsteps;
     for (uint32_t i = 0; i < W; i++) {
         const WorldHost &w = ctx->worlds.at(ctx->order[i]);
         WorldCtr c;
         c.t = (uint32_t)w.t;
         c.remaining = std::min(w.budget, nsteps);
-        ctx->ctr_host[ctx->cur * W + i] = c;
+        ctx->all_active = std::min(ctx->all_active, c.remaining);
+        ctx->ctr_host[ctx->cpar * W + i] = c;
         bs += (uint64_t)w.n * c.remaining;
     }
-    if (W) HIP_TRY(hipMemcpyAsync(ctx->ctr.p + ctx->cur * W, ctx->ctr_host.data() + ctx->cur * W,
+    if (W) HIP_TRY(hipMemcpyAsync(ctx->ctr.p + ctx->cpar * W, ctx->ctr_host.data() + ctx->cpar * W,
                                   W * sizeof(WorldCtr), hipMemcpyHostToDevice, ctx->stream));
     if (bead_steps) *bead_steps = bs;
     return PDPD_OK;
@@ -470,19 +499,47 @@ void finish_counters(pdpd_ctx *ctx, uint32_t nsteps)
     }
 }
 
-int launch_direct(pdpd_ctx *ctx, uint32_t nsteps, KernelEvents *ev)
+// A chain of len steps from buffer c: the first step integrates with
+// k_integrate, the pair-force kernel of every other step's predecessor
+// integrates it; every step sorts from buffer c into c ^ 1, so the chain ends
+// in c ^ 1. The step counters alternate every step. With ev, each step is
+// timed and waited for (profiling).
+hipError_t launch_chain(const pdpd_ctx *ctx, int &c, int &cp, uint32_t len, KernelEvents *ev, double *acc)
 {
-    for (uint32_t s = 0; s < nsteps; s++) {
-        HIP_TRY(launch_step(ctx->step_launch(ctx->cur), ctx->stream, ev));
-        ctx->cur ^= 1;
+    for (uint32_t i = 0; i < len; i++) {
+        hipError_t e = launch_step(ctx->step_launch(c, cp, i == 0, i + 1 < len), ctx->stream, ev);
+        if (e != hipSuccess) return e;
+        cp ^= 1;
+        if (ev) {
+            e = hipEventSynchronize(ev->ev[6]);
+            if (e != hipSuccess) return e;
+            for (int k = 0; k < 6; k++) {
+                float ms = 0;
+                e = hipEventElapsedTime(&ms, ev->ev[k], ev->ev[k + 1]);
+                if (e != hipSuccess) return e;
+                acc[k] += ms;
+            }
+        }
     }
+    if (len) c ^= 1;
+    return hipSuccess;
+}
+
+// nsteps steps from the run's step `first` on: one chain while every world
+// steps, then single steps (a step budget runs out in them).
+int launch_direct(pdpd_ctx *ctx, uint32_t first, uint32_t nsteps)
+{
+    const uint32_t chained =
+        ctx->fuse() && first < ctx->all_active ? std::min(nsteps, ctx->all_active - first) : 0;
+    HIP_TRY(launch_chain(ctx, ctx->cur, ctx->cpar, chained, nullptr, nullptr));
+    for (uint32_t k = chained; k < nsteps; k++) HIP_TRY(launch_chain(ctx, ctx->cur, ctx->cpar, 1, nullptr, nullptr));
     return PDPD_OK;
 }
 
 int ensure_graph(pdpd_ctx *ctx, uint32_t len)
 {
     if (ctx->graph_exec && ctx->graph_version == ctx->layout_version && ctx->graph_cur == ctx->cur &&
-        ctx->graph_len == len)
+        ctx->graph_cpar == ctx->cpar && ctx->graph_len == len)
         return PDPD_OK;
     if (ctx->graph_exec) {
         HIP_TRY(hipGraphExecDestroy(ctx->graph_exec));
@@ -490,16 +547,18 @@ int ensure_graph(pdpd_ctx *ctx, uint32_t len)
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int c = ctx->cur;
-    for (uint32_t s = 0; s < len; s++) {
-        hipError_t e = launch_step(ctx->step_launch(c), ctx->stream, nullptr);
+    // two chains of len / 2 steps (len is even), or len single steps: buffers
+    // and counters end where they started, so the graph replays back to back
+    int c = ctx->cur, cp = ctx->cpar;
+    const bool f = ctx->fuse();
+    for (uint32_t h = 0; h < (f ? 2u : len); h++) {
+        hipError_t e = launch_chain(ctx, c, cp, f ? len / 2 : 1u, nullptr, nullptr);
         if (e != hipSuccess) {
             hipGraph_t junk = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &junk);
             if (junk) (void)hipGraphDestroy(junk);
             return fail(PDPD_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
         }
-        c ^= 1;
     }
     HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
     hipError_t e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
@@ -507,6 +566,7 @@ int ensure_graph(pdpd_ctx *ctx, uint32_t len)
     if (e != hipSuccess) return fail(PDPD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     ctx->graph_version = ctx->layout_version;
     ctx->graph_cur = ctx->cur;
+    ctx->graph_cpar = ctx->cpar;
     ctx->graph_len = len;
     return PDPD_OK;
 }
@@ -515,17 +575,19 @@ int ensure_graph(pdpd_ctx *ctx, uint32_t len)
 // then single steps.
 int enqueue_steps(pdpd_ctx *ctx, uint32_t nsteps)
 {
-    uint32_t left = nsteps;
+    uint32_t done = 0;
     const uint32_t G = ctx->graph_steps & ~1u;
-    if (G >= 2 && left >= G) {
+    // graphs only over steps in which every world steps (chains must not
+    // span the end of a step budget)
+    if (G >= 2 && ctx->all_active >= G) {
         int rc = ensure_graph(ctx, G);
         if (rc) return rc;
-        while (left >= G) {
+        while (ctx->all_active - done >= G) {
             HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-            left -= G;
+            done += G;
         }
     }
-    return launch_direct(ctx, left, nullptr);
+    return launch_direct(ctx, done, nsteps - done);
 }
 
 int check_flags(pdpd_ctx *ctx)
@@ -985,6 +1047,13 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         ctx->graph_version = ~0ull;
         return PDPD_OK;
     }
+    if (k == "fused_integrate") {
+        uint32_t n;
+        if (parse_u32(v, 0, 2, &n)) return fail(PDPD_ERR_INVALID, "fused_integrate must be 0, 1 or 2");
+        ctx->fuse_mode = n;
+        ctx->graph_version = ~0ull;
+        return PDPD_OK;
+    }
     if (k == "force_variant") {
         uint32_t n;
         if (parse_u32(v, 0, kForceVariants - 1, &n))
@@ -1004,6 +1073,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     else if (k == "force_grid") v = std::to_string(ctx->force_grid);
     else if (k == "force_variant") v = std::to_string(ctx->force_variant);
     else if (k == "scan_lookback_tiles") v = std::to_string(ctx->lookback_tiles);
+    else if (k == "fused_integrate") v = std::to_string(ctx->fuse_mode);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -1021,6 +1091,9 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "force_variant=" + std::to_string(ctx->force_variant) +
          "\tpair-force kernel variant: 0 = 3 candidate loads in flight, bonded sums after the middle plane's pairs;"
          " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave\n";
+    s += "fused_integrate=" + std::to_string(ctx->fuse_mode) +
+         "\tchained steps, the pair-force kernel integrating the next step: 0 = off, 1 = on, 2 = on up to " +
+         std::to_string(kFuseAutoBeads) + " resident beads\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }
@@ -1045,15 +1118,12 @@ int pdpd_run_profiled(pdpd_ctx *ctx, uint32_t nsteps)
     KernelEvents ev;
     for (int k = 0; k < 7; k++) HIP_TRY(hipEventCreate(&ev.ev[k]));
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t s = 0; s < nsteps; s++) {
-        HIP_TRY(launch_step(ctx->step_launch(ctx->cur), ctx->stream, &ev));
-        ctx->cur ^= 1;
-        HIP_TRY(hipEventSynchronize(ev.ev[6]));
-        for (int k = 0; k < 6; k++) {
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, ev.ev[k], ev.ev[k + 1]));
-            acc[k] += ms;
-        }
+    // chains as a replayed graph runs them, while every world steps
+    const uint32_t chain = ctx->fuse() ? std::max<uint32_t>(1, (ctx->graph_steps & ~1u) / 2) : 1u;
+    for (uint32_t s = 0; s < nsteps;) {
+        const uint32_t len = s < ctx->all_active ? std::min(chain, ctx->all_active - s) : 1;
+        HIP_TRY(launch_chain(ctx, ctx->cur, ctx->cpar, len, &ev, acc));
+        s += len;
     }
     for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev.ev[k]);
     finish_counters(ctx, nsteps);
@@ -1077,7 +1147,8 @@ int pdpd_time_force(pdpd_ctx *ctx, uint32_t reps, double *avg_ms)
     if (ctx->dirty || ctx->n_worlds == 0 || ctx->steps_on_layout == 0)
         return fail(PDPD_ERR_STATE, "time_force needs a completed step on the current layout");
     reps = std::max<uint32_t>(1, reps);
-    const StepLaunch L = ctx->step_launch(ctx->cur ^ 1);   // the step that produced the current buffers
+    // the step that produced the current buffers (the last of a chain)
+    const StepLaunch L = ctx->step_launch(ctx->cur ^ 1, ctx->cpar ^ 1, false, false);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1181,7 +1252,7 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     HIP_TRY(dev_alloc(sv, nb));
     HIP_TRY(dev_alloc(sf, nb));
     HIP_TRY(dev_alloc(ss, nb));
-    const int c0 = ctx->cur;
+    const int c0 = ctx->cur, cp0 = ctx->cpar;
     // save and restore on the engine's stream: the stream is non-blocking, so a
     // plain hipMemcpy would not be ordered with the trial steps
     HIP_TRY(hipMemcpyAsync(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1195,6 +1266,7 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
         HIP_TRY(hipMemcpyAsync(ctx->slot_of_bead.p, ss.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                                ctx->stream));
         ctx->cur = c0;
+        ctx->cpar = cp0;
         return PDPD_OK;
     };
     int cus = 256;
@@ -1206,14 +1278,17 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     const uint32_t grids[] = {0u, (uint32_t)cus * 4, (uint32_t)cus * 8, (uint32_t)cus * 16};
     const uint32_t graphs[] = {10u, 20u, 50u};
     uint32_t best_grid = ctx->force_grid, best_graph = ctx->graph_steps, best_var = ctx->force_variant;
+    const uint32_t fuse0 = ctx->fuse() ? 1u : 0u;
+    uint32_t best_fuse = fuse0;
     double best_ms = 1e30;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    auto trial = [&](uint32_t g, uint32_t var, uint32_t gs) -> int {
+    auto trial = [&](uint32_t g, uint32_t var, uint32_t gs, uint32_t fz) -> int {
         ctx->force_grid = g;
         ctx->force_variant = var;
         ctx->graph_steps = gs;
+        ctx->fuse_mode = fz;
         ctx->graph_version = ~0ull;
         const uint32_t steps = std::max(steps_per_trial, gs);
         for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
@@ -1228,16 +1303,18 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         const double per = ms / steps;
-        if (per < best_ms) { best_ms = per; best_grid = g; best_var = var; best_graph = gs; }
+        if (per < best_ms) { best_ms = per; best_grid = g; best_var = var; best_graph = gs; best_fuse = fz; }
         return PDPD_OK;
     };
-    // pair-force grid x kernel variant at 20 steps per graph, then the graph length
+    // pair-force grid x kernel variant at 20 steps per graph, then the graph
+    // length with chained steps off and on
     for (uint32_t g : grids)
         for (uint32_t var = 0; var < kForceVariants; var++)
-            if ((rc = trial(g, var, 20u))) return rc;
+            if ((rc = trial(g, var, 20u, fuse0))) return rc;
     const uint32_t g_best = best_grid, v_best = best_var;
     for (uint32_t gs : graphs)
-        if (gs != 20u && (rc = trial(g_best, v_best, gs))) return rc;
+        for (uint32_t fz = 0; fz < 2; fz++)
+            if ((gs != 20u || fz != fuse0) && (rc = trial(g_best, v_best, gs, fz))) return rc;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if ((rc = restore())) return rc;
@@ -1247,11 +1324,12 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     ctx->force_grid = best_grid;
     ctx->force_variant = best_var;
     ctx->graph_steps = best_graph;
+    ctx->fuse_mode = best_fuse;
     ctx->graph_version = ~0ull;
     if ((rc = check_flags(ctx))) return rc;
     if (choice && choice_len)
-        snprintf(choice, choice_len, "force_grid=%u force_variant=%u graph_steps=%u ms_per_step=%.5f", best_grid,
-                 best_var, best_graph, best_ms);
+        snprintf(choice, choice_len, "force_grid=%u force_variant=%u graph_steps=%u fused_integrate=%u ms_per_step=%.5f",
+                 best_grid, best_var, best_graph, best_fuse, best_ms);
     return PDPD_OK;
 }
 

@@ -78,11 +78,10 @@ struct DeviceState {
     float4 *pos[2];        // x, y, z, hash bits
     float4 *vel[2];        // w = v - f dt/2 (x,y,z), global bead index bits
     float4 *frc[2];        // fx, fy, fz, 0
-    uint32_t *key;         // cell id per slot after the drift
-    uint32_t *rank;        // arrival rank inside the cell | inactive bit
+    uint32_t *key;         // cell id per slot after the drift | inactive bit
     uint32_t *perm;        // sorted position -> old slot | inactive bit
     uint32_t *sorted_idx;  // sorted position -> global bead index
-    uint32_t *cell_count;  // n_cells_total
+    uint32_t *cell_count;  // n_cells_total: counts; after the scan, the scatter's arrival cursors
     uint32_t *cell_start;  // n_cells_total + 1
     unsigned long long *scan_tiles;   // look-back state per scan tile
     uint32_t *scan_ticket;
@@ -101,6 +100,18 @@ struct DeviceState {
     // type pair (home type * n_types + other type), (a, gamma) in FP32. A pair
     // uses the mean of its two cells' values.
     const float2 *cell_par;
+};
+
+// The buffers the pair-force kernel writes when it integrates the next step
+// of a chain: for a step sorting from buffer c with counter copy cp, buffer c
+// and counter copy cp ^ 1. One entry per (c, cp), in device memory.
+struct NextBuffers {
+    float4 *pos, *vel, *frc;
+    uint32_t *key, *cell_count, *flags;
+    unsigned long long *scan_tiles;
+    uint32_t *scan_ticket;
+    const WorldCtr *ctr;
+    uint32_t n_tiles;
 };
 
 }  // namespace pdpd

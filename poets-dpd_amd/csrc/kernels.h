@@ -9,7 +9,11 @@ namespace pdpd {
 // Everything one step needs to launch.
 struct StepLaunch {
     DeviceState S;
-    int cur;                 // buffer (and counter copy) holding the state at step start
+    int cur;                 // buffer holding the state at step start (the sort writes cur ^ 1)
+    int cpar;                // counter copy this step reads (the next step's is cpar ^ 1)
+    int integrate;           // run k_integrate (the first step of a chain)
+    int fuse;                // the pair-force kernel integrates the next step (all but a chain's last step)
+    const NextBuffers *next; // device table of 4 entries, [cur * 2 + cpar]
     uint32_t n_beads;
     uint32_t n_cells;
     uint32_t n_worlds;

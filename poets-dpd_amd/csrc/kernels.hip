@@ -1,14 +1,16 @@
 // kernels.hip -- gfx950 kernels of one DPD time step.
 //
 // Step (one set of launches advances every resident world):
-//   k_integrate   finish-kick + drift + half-kick + wrap, cell key, arrival rank
+//   k_integrate   finish-kick + drift + half-kick + wrap, cell key, cell counts
 //                 (dpd_maths_core_half_step_raw.hpp:42-72,114-124 fused as
-//                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does); advances
-//                 the per-world step counters
+//                 dpd_engine_avx2_half_merge_tbb.hpp:780-800 does); only the
+//                 first step of a chain: later steps are integrated by the
+//                 pair-force kernel of the step before them
 //   k_scan        single-pass exclusive scan of the cell counts (look-back)
-//   k_scatter     sorted position of every bead's slot; extra workgroups
-//                 evaluate the bonded terms (one thread per Hookean bond / angle,
-//                 FP64; terms of naive_dpd_engine_half_merge.hpp:638-690 /
+//   k_scatter     arrival position of every bead in its cell; advances the
+//                 per-world step counters; extra workgroups evaluate the bonded
+//                 terms (one thread per Hookean bond / angle, FP64; terms of
+//                 naive_dpd_engine_half_merge.hpp:638-690 /
 //                 dpd_maths_core_half_step.hpp:178-231), once per term
 //   k_reorder     stable gather into the other buffer, bead-index order in a cell
 //   k_force       per workgroup 64 consecutive beads: three waves for the pair
@@ -18,10 +20,11 @@
 //                 hash.hpp:61-88 + dpd_maths_core.hpp:55-79); the home beads'
 //                 bonded terms summed in FP64 in the per-polymer order of
 //                 naive_dpd_engine_half_merge.hpp:376-385 (by a fourth wave or
-//                 after the middle plane's pairs)
+//                 after the middle plane's pairs); within a chain, then the
+//                 next step's integration of its beads
 //   k_stationary  (only with stationary bead types) v = 0 after the pair forces
-// The final half-kick v += f dt/2 of a step is folded into the next k_integrate
-// (the engine stores w = v - f dt/2).
+// The final half-kick v += f dt/2 of a step is folded into the next step's
+// integration (the engine stores w = v - f dt/2).
 //
 // Arithmetic is FP32, built with -ffp-contract=off: every fused multiply-add is
 // an explicit fmaf, sqrtf and '/' are IEEE correctly rounded. The CPU oracle
@@ -132,86 +135,103 @@ __device__ __forceinline__ int cell_coord(float x, int L)
 
 // ---------------------------------------------------------- integrate ----
 
-__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur, uint32_t n_tiles, uint32_t n_blocks,
-                                                           WorldCtr *__restrict__ ctr_next, uint32_t n_worlds)
+// One bead's finish-kick, drift, half-kick and wrap from its force f:
+// v(t) = w + f dt/2, v(t+dt/2) = v(t) + f dt/2, x(t+dt) = x + dt v(t+dt/2),
+// then w = v(t+dt/2). A bead of a stationary type keeps x; the pair force sees
+// v(t) (the imported v on the first step, 0 after) --
+// naive_dpd_engine_half_merge.hpp:327-331.
+__device__ __forceinline__ void drift_bead(const WorldDev *__restrict__ W, float4 &p, float4 &v, const float4 f)
 {
-    if (blockIdx.x == 0) {       // reset the scan's look-back state for this step
-        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x) S.scan_tiles[i] = 0ull;
-        if (threadIdx.x == 0) *S.scan_ticket = 0u;
-    }
-    {
-        // step counters: this step reads S.ctr; the next one reads ctr_next,
-        // which nothing reads during this step
-        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i < n_worlds) {
-            const WorldCtr c = S.ctr[i];
-            WorldCtr o = c;
-            if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
-            ctr_next[i] = o;
+    const float dt = W->dt, hdt = W->hdt;
+    const float fv[3] = {f.x, f.y, f.z};
+    float x[3] = {p.x, p.y, p.z};
+    float wv[3] = {v.x, v.y, v.z};
+    if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) wv[d] = fmaf(fv[d], hdt, wv[d]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float Lf = W->Lf[d];
+            float vv = fmaf(fv[d], hdt, wv[d]);     // v(t)       = w + f dt/2
+            float vh = fmaf(fv[d], hdt, vv);        // v(t+dt/2)  = v + f dt/2
+            float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
+            xn += (xn < 0.0f) ? Lf : 0.0f;
+            xn -= (xn >= Lf) ? Lf : 0.0f;
+            x[d] = xn;
+            wv[d] = vh;
         }
     }
+    p = make_float4(x[0], x[1], x[2], p.w);
+    v = make_float4(wv[0], wv[1], wv[2], v.w);
+}
+
+// Runs of equal keys in consecutive lanes of one wave (called converged; the
+// valid lanes are a prefix and hold consecutive slots in the last step's cell
+// order, so beads of one cell come in runs): for each valid lane, the first
+// lane of its run (start) and the run's length from this lane on.
+__device__ __forceinline__ void key_runs(uint32_t key, bool valid, int &start, int &next, bool &head)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long lanes = __ballot(valid);
+    const unsigned long long upto = (2ull << lane) - 1ull;          // lanes 0..lane
+    start = 63 - __clzll((long long)(heads & upto));                 // first lane of this run
+    const unsigned long long above = heads & ~upto;
+    next = above ? __ffsll((long long)above) - 1 : (int)__popcll(lanes);
+}
+
+// The cell key of the bead in slot s at position p (with kInactive for a
+// bead that does not move this step), and its count in cell_count: one
+// atomic per run of equal keys, whose result is not waited for. Raises
+// PDPD_FLAG_BAD_POSITION for a position outside the box. Called by all lanes
+// of a wave, converged.
+__device__ __forceinline__ void count_bead(uint32_t *__restrict__ flags, uint32_t *__restrict__ cell_count,
+                                           uint32_t *__restrict__ key_out, const WorldDev *__restrict__ W, uint32_t s,
+                                           const float4 p, bool active, bool valid)
+{
+    const float x[3] = {p.x, p.y, p.z};
+    bool bad = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++) bad |= !(x[d] >= 0.0f && x[d] < W->Lf[d]);
+    if (valid && bad) atomicOr(flags, PDPD_FLAG_BAD_POSITION);
+    const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
+    const uint32_t key = valid ? W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2) : 0xffffffffu;
+    int start, next;
+    bool head;
+    key_runs(key, valid, start, next, head);
+    if (head) atomicAdd(&cell_count[key], (uint32_t)(next - (int)(threadIdx.x & 63)));
+    if (valid) key_out[s] = key | (active ? 0u : kInactive);
+}
+
+__device__ __forceinline__ void reset_scan_state(const DeviceState &S, uint32_t n_tiles, int tid, int nthreads)
+{
+    for (uint32_t i = (uint32_t)tid; i < n_tiles; i += (uint32_t)nthreads) S.scan_tiles[i] = 0ull;
+    if (tid == 0) *S.scan_ticket = 0u;
+}
+
+// The first step of a chain of steps: integration in place in buffer cur from
+// the forces of the previous step (later steps of a chain are integrated by
+// the pair-force kernel of the step before them).
+__global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int cur, uint32_t n_tiles, uint32_t n_blocks)
+{
+    if (blockIdx.x == 0) reset_scan_state(S, n_tiles, threadIdx.x, blockDim.x);   // for this step's scan
     if (blockIdx.x >= n_blocks) return;
     const BeadBlock blk = S.bead_blocks[blockIdx.x];
     const uint32_t s = blk.begin + threadIdx.x;
     if (s >= blk.end) return;
     const WorldDev *__restrict__ W = &S.worlds[blk.world];
     const bool active = S.ctr[blk.world].remaining > 0;   // S.ctr points at this step's counters
-
     float4 p = S.pos[cur][s];
-    float x[3] = {p.x, p.y, p.z};
     if (active) {
-        const float4 f = S.frc[cur][s];
         float4 v = S.vel[cur][s];
-        const float dt = W->dt, hdt = W->hdt;
-        const float fv[3] = {f.x, f.y, f.z};
-        float wv[3] = {v.x, v.y, v.z};
-        if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) {
-            // stationary type: x stays; the pair force sees v(t) (the imported v on
-            // the first step, 0 after) -- naive_dpd_engine_half_merge.hpp:327-331
-#pragma unroll
-            for (int d = 0; d < 3; d++) wv[d] = fmaf(fv[d], hdt, wv[d]);
-            S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
-        } else {
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const float Lf = W->Lf[d];
-                float vv = fmaf(fv[d], hdt, wv[d]);     // v(t)       = w + f dt/2
-                float vh = fmaf(fv[d], hdt, vv);        // v(t+dt/2)  = v + f dt/2
-                float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
-                xn += (xn < 0.0f) ? Lf : 0.0f;
-                xn -= (xn >= Lf) ? Lf : 0.0f;
-                x[d] = xn;
-                wv[d] = vh;
-            }
-            S.pos[cur][s] = make_float4(x[0], x[1], x[2], p.w);
-            S.vel[cur][s] = make_float4(wv[0], wv[1], wv[2], v.w);
-        }
+        drift_bead(W, p, v, S.frc[cur][s]);
+        S.pos[cur][s] = p;
+        S.vel[cur][s] = v;
     }
-    bool bad = false;
-#pragma unroll
-    for (int d = 0; d < 3; d++) bad |= !(x[d] >= 0.0f && x[d] < W->Lf[d]);
-    if (bad) atomicOr(S.flags, PDPD_FLAG_BAD_POSITION);
-    const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
-    const uint32_t key = W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2);
-    // arrival rank in the cell. The lanes hold beads in the last step's cell
-    // order, so equal keys come in runs of consecutive lanes: one atomic per
-    // run reserves the run's ranks (active lanes are a prefix of the wave).
-    const int lane = threadIdx.x & 63;
-    const uint32_t prev = __shfl_up(key, 1, 64);
-    const bool head = lane == 0 || prev != key;
-    // both ballots in converged code: a ballot inside a branch sees only the
-    // lanes that take the branch
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long lanes = __ballot(1);
-    const unsigned long long upto = (2ull << lane) - 1ull;          // lanes 0..lane
-    const int start = 63 - __clzll((long long)(heads & upto));       // first lane of this run
-    const unsigned long long above = heads & ~upto;
-    const int next = above ? __ffsll((long long)above) - 1 : (int)__popcll(lanes);
-    uint32_t base = 0;
-    if (head) base = atomicAdd(&S.cell_count[key], (uint32_t)(next - lane));
-    const uint32_t r = __shfl(base, start, 64) + (uint32_t)(lane - start);
-    S.key[s] = key;
-    S.rank[s] = r | (active ? 0u : kInactive);
+    count_bead(S.flags, S.cell_count, S.key, W, s, p, active, true);   // lanes past the block's end have returned
 }
 
 // --------------------------------------------------------------- scan ----
@@ -293,7 +313,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ co
         const uint32_t i = base + k;
         if (i < n) {
             cell_start[i] = ex;
-            count[i] = 0u;          // ready for the next step's arrivals
+            count[i] = ex;          // the scatter's arrival cursor (k_reorder clears it)
         }
         ex += v[k];
     }
@@ -365,7 +385,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
         const uint32_t i = base + k;
         if (i < n) {
             cell_start[i] = ex;
-            count[i] = 0u;          // ready for the next step's arrivals
+            count[i] = ex;          // the scatter's arrival cursor (k_reorder clears it)
         }
         ex += v[k];
     }
@@ -378,14 +398,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
 // slot_of_bead is read only for bonded beads (the bonded terms) and stationary
 // ones; monomers (BeadHash bit 27) have no bonds, so their entries are not
 // written unless stationary beads are resident.
-__global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur, uint32_t n, int all_slots)
+__global__ __launch_bounds__(kBeadThreads) void k_reorder(DeviceState S, int cur, uint32_t n, uint32_t n_cells,
+                                                          int all_slots)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_cells) S.cell_count[j] = 0u;          // ready for the next step's counts
     if (j >= n) return;
     const int nxt = cur ^ 1;
     const uint32_t e = S.perm[j];
     const uint32_t s = e & ~kInactive;
-    const uint32_t k = S.key[s];
+    const uint32_t k = S.key[s] & ~kInactive;
     const uint32_t a = S.cell_start[k], b = S.cell_start[k + 1];
     const uint32_t me = S.sorted_idx[j];
     uint32_t rank = 0;
@@ -466,10 +488,22 @@ __device__ __forceinline__ void eval_term(const DeviceState &S, int cur, uint32_
 }
 
 // Sort scatter: the sorted position of every bead's slot (n_scatter_blocks
-// workgroups), then the bonded terms (the remaining workgroups).
+// workgroups), then the bonded terms (the remaining workgroups). Also advances
+// the step counters: this step reads S.ctr; the next one reads ctr_next, which
+// nothing of this step reads before the pair-force kernel.
 __global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur, uint32_t n,
-                                                          uint32_t n_scatter_blocks, uint32_t n_terms)
+                                                          uint32_t n_scatter_blocks, uint32_t n_terms,
+                                                          WorldCtr *__restrict__ ctr_next, uint32_t n_worlds)
 {
+    {
+        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n_worlds) {
+            const WorldCtr c = S.ctr[i];
+            WorldCtr o = c;
+            if (c.remaining > 0) { o.t = c.t + 1; o.remaining = c.remaining - 1; }
+            ctr_next[i] = o;
+        }
+    }
     if (blockIdx.x >= n_scatter_blocks) {
         const uint32_t k = (blockIdx.x - n_scatter_blocks) * blockDim.x + threadIdx.x;
         if (k < n_terms) eval_term(S, cur, k);
@@ -477,9 +511,18 @@ __global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur
     }
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const uint32_t r = S.rank[s];
-    const uint32_t j = S.cell_start[S.key[s]] + (r & ~kInactive);
-    S.perm[j] = s | (r & kInactive);
+    // arrival position in the cell: each run of equal keys in consecutive
+    // lanes reserves its positions with one atomic on the cell's cursor
+    // (cell_count holds the cell's start after the scan)
+    const uint32_t k = S.key[s];
+    const uint32_t key = k & ~kInactive;
+    int start, next;
+    bool head;
+    key_runs(key, true, start, next, head);                // lanes past the end have returned
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&S.cell_count[key], (uint32_t)(next - (int)(threadIdx.x & 63)));
+    const uint32_t j = __shfl(base, start, 64) + (uint32_t)((int)(threadIdx.x & 63) - start);
+    S.perm[j] = s | (k & kInactive);
     S.sorted_idx[j] = fbits(S.vel[cur][s].w);
 }
 
@@ -577,6 +620,39 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The next step's integration, done by the pair-force kernel for every step of
+// a chain but the last. From the position, half-step velocity and the force
+// just computed, each bead's drifted state goes into buffer nxt ^ 1 (the
+// step's source buffer, free after the sort) at the same slot, with its cell
+// key and count: what k_integrate would compute at the start of the
+// next step, so the next step starts at its scan. Chains are only formed over
+// steps in which every resident world steps (no step budget runs out inside
+// a chain but at its end), so every bead is active in both steps.
+// The kernel reads the next step's buffers through an entry of a table in
+// device memory (null: not chained), after its pair loop, so that their
+// addresses take no scalar registers during the loop.
+
+// One bead (all lanes of the wave call it, the valid ones a prefix).
+__device__ __forceinline__ void advance_next(const NextBuffers &B, const WorldDev *__restrict__ W, uint32_t s,
+                                             bool valid, float4 p, float4 v, const float4 f)
+{
+    if (valid) {
+        // k_stationary zeroes the velocity of a stationary bead after this kernel
+        if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) v = make_float4(0.0f, 0.0f, 0.0f, v.w);
+        drift_bead(W, p, v, f);
+        B.pos[s] = p;
+        B.vel[s] = v;
+    }
+    count_bead(B.flags, B.cell_count, B.key, W, s, p, true, valid);
+}
+
+// The table entry, read after a compiler barrier so the loads stay here.
+__device__ __forceinline__ NextBuffers next_buffers(const NextBuffers *b)
+{
+    __asm__ volatile("" ::: "memory");
+    return *b;
+}
+
 // Pair forces. A workgroup takes 64 consecutive slots of one world (one home
 // bead per lane) and has one wave per x-plane of the 27-cell stencil. A wave
 // scans the three rows (dy = -1, 0, +1) of its plane in turn; per row the
@@ -600,7 +676,8 @@ __device__ __forceinline__ void wave_sync()
 // other type; other worlds use the constant tables as without it.
 template <int kScanWidth, int kWaves, bool kBondWave, bool kSpatial>
 __global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
-__attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks)
+__attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks,
+                                                           const NextBuffers *next)
 {
     // a batch accepts at most 64 * kScanWidth pairs, which an emptied queue must hold
     static_assert(kQueue >= kForceBeads * kScanWidth && kQueue <= 256, "queue positions are 8-bit");
@@ -615,6 +692,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
     // per home bead and plane: its queued pairs in acceptance order
     __shared__ uint8_t s_list[kLaneList][kPairLanes];
     __shared__ float4 s_hp[kForceBeads], s_hv[kForceBeads];
+    __shared__ uint32_t s_world;
     __shared__ float s_part[kForcePlanes - 1][3][kForceBeads];
     __shared__ double s_bond[3][kForceBeads];
     __shared__ uint32_t s_nterms[kForceBeads];
@@ -626,11 +704,17 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
     const unsigned long long below = (1ull << lane) - 1ull;
     const float4 *__restrict__ pos = S.pos[nxt];
     const float4 *__restrict__ vel = S.vel[nxt];
+    float4 *__restrict__ frc = S.frc[nxt];
     const uint32_t *__restrict__ cs = S.cell_start;
+    if (next && blockIdx.x == 0) {                       // for the next step's scan
+        const NextBuffers B = next_buffers(next);
+        for (uint32_t i = (uint32_t)tid; i < B.n_tiles; i += kThreads) B.scan_tiles[i] = 0ull;
+        if (tid == 0) *B.scan_ticket = 0u;
+    }
 
     for (uint32_t li = blockIdx.x; li < n_blocks; li += gridDim.x) {
         const BeadBlock blk = S.force_blocks[xcd_chunk(li, n_blocks)];
-        if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world
+        if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world (never in a chain)
         const WorldDev *__restrict__ W = &S.worlds[blk.world];
         for (int i = tid; i < kMaxTypes * kMaxTypes; i += kThreads) {
             s_a[i] = W->a[i];
@@ -647,6 +731,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
         if (plane == 0) {
             s_hp[lane] = hp;
             s_hv[lane] = vel[sl];
+            if (lane == 0) s_world = blk.world;
         }
         __syncthreads();
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -850,9 +935,15 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 fy = (float)((double)fy + s_bond[1][lane]);
                 fz = (float)((double)fz + s_bond[2][lane]);
             }
-            S.frc[nxt][s] = make_float4(fx, fy, fz, 0.0f);
+            frc[s] = make_float4(fx, fy, fz, 0.0f);
         }
         __syncthreads();
+        if (next && plane == 0) {          // the whole wave, converged; f as stored above by this lane
+            const NextBuffers B = next_buffers(next);          // (after its compiler barrier: reloaded here)
+            const uint32_t s2 = blk.begin + lane;
+            advance_next(B, &S.worlds[s_world], s2, live, s_hp[lane], s_hv[lane],
+                         live ? frc[s2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        }
     }
 }
 
@@ -987,16 +1078,14 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
 {
     const DeviceState &S = L.S;
     const int cur = L.cur;
-    const int par = L.cur;   // counters ping-pong with the buffers
     DeviceState Sp = S;
-    Sp.ctr = S.ctr + par * L.n_worlds;       // counters read this step
+    Sp.ctr = S.ctr + L.cpar * L.n_worlds;       // counters read this step
     auto rec = [&](int k) { if (ev) (void)hipEventRecord(ev->ev[k], st); };
     const uint32_t nt = scan_tiles(L.n_cells);
 
     rec(0);
-    const uint32_t ig = std::max<uint32_t>(std::max<uint32_t>(1, L.n_bead_blocks), ceil_div(L.n_worlds, kBeadThreads));
-    k_integrate<<<ig, kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks, S.ctr + (par ^ 1) * L.n_worlds,
-                                              L.n_worlds);
+    if (L.integrate)
+        k_integrate<<<std::max<uint32_t>(1, L.n_bead_blocks), kBeadThreads, 0, st>>>(Sp, cur, nt, L.n_bead_blocks);
     rec(1);
     if (nt <= L.lookback_tiles) {
         k_scan<<<nt, kScanThreads, 0, st>>>(S.cell_count, L.n_cells, S.cell_start, S.scan_tiles, S.scan_ticket, nt,
@@ -1008,9 +1097,12 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
     }
     rec(2);
     const uint32_t gb = ceil_div(L.n_beads, kBeadThreads);
-    k_scatter<<<gb + ceil_div(L.n_terms, kBeadThreads), kBeadThreads, 0, st>>>(Sp, cur, L.n_beads, gb, L.n_terms);
+    const uint32_t sg = std::max(gb + ceil_div(L.n_terms, kBeadThreads), ceil_div(L.n_worlds, kBeadThreads));
+    k_scatter<<<std::max<uint32_t>(1, sg), kBeadThreads, 0, st>>>(Sp, cur, L.n_beads, gb, L.n_terms,
+                                                                   S.ctr + (L.cpar ^ 1) * L.n_worlds, L.n_worlds);
     rec(3);
-    k_reorder<<<gb, kBeadThreads, 0, st>>>(S, cur, L.n_beads, L.n_stationary > 0 ? 1 : 0);
+    k_reorder<<<std::max<uint32_t>(1, ceil_div(std::max(L.n_beads, L.n_cells), kBeadThreads)), kBeadThreads, 0, st>>>(
+        S, cur, L.n_beads, L.n_cells, L.n_stationary > 0 ? 1 : 0);
     rec(4);
     const hipError_t e = launch_force(L, st);
     if (e != hipSuccess) return e;
@@ -1024,19 +1116,20 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
 hipError_t launch_force(const StepLaunch &L, hipStream_t st)
 {
     DeviceState Sp = L.S;
-    Sp.ctr = L.S.ctr + L.cur * L.n_worlds;       // counters read by that step
+    Sp.ctr = L.S.ctr + L.cpar * L.n_worlds;      // counters read by that step
     const int nxt = L.cur ^ 1;
+    const NextBuffers *ns = L.fuse ? L.next + (L.cur * 2 + L.cpar) : nullptr;
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     constexpr uint32_t t3 = kForceBeads * kForcePlanes;
     if (fg && L.spatial) {
         // any world with spatially varying strengths: one shape, with the per-cell lookup
-        k_force<3, 4, false, true><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks);
+        k_force<3, 4, false, true><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns);
     } else if (fg) {
         switch (L.force_variant) {
-        case 1: k_force<4, 4, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        case 2: k_force<2, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        case 3: k_force<3, 5, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
-        default: k_force<3, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks); break;
+        case 1: k_force<4, 4, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        case 2: k_force<2, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        case 3: k_force<3, 5, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        default: k_force<3, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         }
     }
     return hipGetLastError();

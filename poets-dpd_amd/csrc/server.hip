@@ -98,7 +98,7 @@ struct Request {
 };
 
 // Device bytes one resident world needs: double-buffered pos/vel/frc (96 B),
-// sort keys, ranks, permutation and bead->slot map (16 B) per bead, the cell
+// sort keys, permutation, sorted indices and bead->slot map (16 B) per bead, the cell
 // counts and starts (8 B) per cell, and per-cell strengths (8 B per cell and
 // type pair) if it has them.
 uint64_t device_bytes(const pdpd_world_desc &d)

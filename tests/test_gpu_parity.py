@@ -232,7 +232,9 @@ def test_autotune_leaves_state_unchanged(gpu_lib):
 @pytest.mark.parametrize("cfg", [dict(graph_steps=0), dict(graph_steps=8), dict(force_grid=64),
                                  dict(force_grid=1024, graph_steps=6),
                                  dict(scan_lookback_tiles=0), dict(scan_lookback_tiles=0, graph_steps=4),
-                                 dict(force_variant=1), dict(force_variant=2), dict(force_variant=3)])
+                                 dict(force_variant=1), dict(force_variant=2), dict(force_variant=3),
+                                 dict(fused_integrate=0), dict(fused_integrate=1, graph_steps=6),
+                                 dict(fused_integrate=1, graph_steps=0, force_grid=64)])
 def test_launch_shapes_bitwise_identical(gpu_lib, cfg):
     w = W.make_world((12, 10, 9), bonded_fraction=0.1, seed=41, a_matrix=W.random_a_matrix(np.random.default_rng(4)))
     _say(f"launch shape {cfg}")

@@ -87,6 +87,24 @@ for slot, w in zip(slots, (worlds["spatial"], worlds["water4"])):
              np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
 print("mixed", "bitwise" if same else "DIFFERENT", flush=True)
 ctx.close()
+# chains of steps (each step's pair-force kernel integrating the next): a
+# stationary wall whose budget ends mid-chain beside a world that runs on,
+# direct launches and a replayed graph
+for gs in (0, 4):
+    ctx = capi.Context(0)
+    ctx.set_config("graph_steps", gs)
+    a = ctx.add(worlds["wall_bent6"]); b = ctx.add(worlds["poly_6x7x8"])
+    ctx.set_steps(a, 7)
+    ctx.run(11)                 # a graph of 4 (two chains of 2), a chain of 3, then 4 single steps
+    same = True
+    for slot, w, n in ((a, worlds["wall_bent6"], 7), (b, worlds["poly_6x7x8"], 11)):
+        x, ww, f, t = ctx.download_f32(slot)
+        ref = O.F32State(w).step(n)
+        same &= (t == n and np.array_equal(x.view(np.uint32), ref.x.view(np.uint32)) and
+                 np.array_equal(ww.view(np.uint32), ref.w.view(np.uint32)) and
+                 np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
+    print(f"chain_graph{gs}", "bitwise" if same else "DIFFERENT", flush=True)
+    ctx.close()
 """
 
 
@@ -96,7 +114,7 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 7, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 9, r.stdout
 
 
 SERVE_PROBE = r"""
