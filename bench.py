@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--side", type=int, default=32)
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
+    ap.add_argument("--fused-integrate", type=int, default=None, choices=(0, 1, 2),
+                    help="chained steps: 0 off, 1 on, 2 automatic (the engine's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB, 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -210,6 +212,8 @@ def main():
     slot = ctx.add(world)
     if args.graph_steps is not None:
         ctx.set_config("graph_steps", str(args.graph_steps))
+    if args.fused_integrate is not None:
+        ctx.set_config("fused_integrate", str(args.fused_integrate))
     if args.autotune:
         log("bench: autotune " + ctx.autotune(20))
     ctx.run(args.warmup)
@@ -237,6 +241,7 @@ def main():
                 "kernel_ms_per_step": {k[2:-3]: st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms",
                                                                "k_reorder_ms", "k_force_ms", "k_stationary_ms")}}
     x, v, f, t = ctx.download(slot)
+    fused = ctx.get_config("fused_integrate")
     ctx.close()
 
     out = {"metric": METRIC, "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -244,7 +249,7 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"plain engine, polymer world {args.side}x{args.side}x{args.side} cells, "
                                   f"density 3, {n} beads, 10% in H2T4 chains (Hookean k=128 r0=0.5, angle k=20)",
-                      "beads": n, "dt": 0.01, "engine": "HipDPDEngine (graph-captured steps)"},
+                      "beads": n, "dt": 0.01, "engine": f"HipDPDEngine (graph-captured steps, fused_integrate={fused})"},
            "roofline": roofline}
     if args.batch:
         log(f"bench: batched ensemble of {args.batch} worlds")
