@@ -181,7 +181,7 @@ struct pdpd_ctx {
     uint32_t graph_steps = 20;
     uint32_t force_grid = 0;
     uint32_t force_variant = 0;
-    uint32_t lookback_tiles = 64;
+    uint32_t lookback_tiles = 256;
     uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
 
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
@@ -1087,7 +1087,7 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "force_grid=" + std::to_string(ctx->force_grid) +
          "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
     s += "scan_lookback_tiles=" + std::to_string(ctx->lookback_tiles) +
-         "\tcell scan: single pass with look-back up to this many 2048-cell tiles, three phases above\n";
+         "\tcell scan: single pass with look-back up to this many 512-cell tiles, three phases above\n";
     s += "force_variant=" + std::to_string(ctx->force_variant) +
          "\tpair-force kernel variant: 0 = 3 candidate loads in flight, bonded sums after the middle plane's pairs;"
          " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave\n";

@@ -13,7 +13,7 @@ constexpr int kForceBeads = 64;        // beads per pair-force workgroup (one pe
 constexpr int kForcePlanes = 3;        // pair waves per pair-force workgroup: one per x-plane of the stencil
 constexpr int kForceThreads = kForceBeads * (kForcePlanes + 1);   // + one wave for the bonded terms
 constexpr int kScanThreads = 256;
-constexpr int kScanPerThread = 8;
+constexpr int kScanPerThread = 2;
 constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan tile
 constexpr uint32_t kMaxLayoutBeads = 1u << 26;              // pair-list entries carry 26-bit slots
 
