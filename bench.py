@@ -88,13 +88,15 @@ def cpu_baseline(world, target_s):
     return out
 
 
-def run_batched(args, ctx_cls, W):
-    """Mixed-size ensemble: all worlds advance in one launch set per step, vs the
-    same worlds one at a time through the plain engine."""
+def run_batched(args, capi, W):
+    """Mixed-size ensemble: all worlds advance together (one launch set per step
+    in each of the ensemble's contexts, whose streams overlap), vs the same
+    worlds one at a time through the plain engine."""
     worlds = W.config_batch(n_worlds=args.batch, seed=1)
     total = sum(w.n_beads for w in worlds)
     steps = args.batch_steps
-    ctx = ctx_cls(0)
+    parts = args.batch_parts if args.batch_parts else capi.suggested_parts(total)
+    ctx = capi.Ensemble(0, parts)
     for w in worlds:
         ctx.add(w)
     tune = ctx.autotune(10) if not args.no_batch_autotune else "heuristic"
@@ -111,7 +113,7 @@ def run_batched(args, ctx_cls, W):
     solo_time = 0.0
     solo_beads = 0
     for w in worlds[:k]:
-        c = ctx_cls(0)
+        c = capi.Context(0)
         c.add(w)
         c.run(2)
         c.sync()
@@ -123,7 +125,8 @@ def run_batched(args, ctx_cls, W):
         c.close()
     solo = solo_beads * steps / solo_time
     return {"worlds": len(worlds), "beads": total, "steps": steps, "value": batched, "unit": "bead-steps/s",
-            "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "tuning": tune}
+            "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "contexts": parts,
+            "tuning": tune}
 
 
 def run_served(args, capi, W):
@@ -178,6 +181,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="batched ensemble of this many worlds (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-solo-sample", type=int, default=16)
+    ap.add_argument("--batch-parts", type=int, default=0,
+                    help="contexts (streams) the ensemble is spread over (0 = the engine's suggestion)")
     ap.add_argument("--autotune", action="store_true", help="autotune the plain engine's launch shape too")
     ap.add_argument("--no-batch-autotune", action="store_true",
                     help="batched ensemble with the default launch shape instead of the autotuner's")
@@ -253,7 +258,7 @@ def main():
            "roofline": roofline}
     if args.batch:
         log(f"bench: batched ensemble of {args.batch} worlds")
-        out["batched"] = run_batched(args, capi.Context, W)
+        out["batched"] = run_batched(args, capi, W)
     if args.serve:
         log(f"bench: served sweep of {args.serve} requests")
         out["served"] = run_served(args, capi, W)

@@ -1,5 +1,6 @@
 // CPU-emulated runs of the engine through its C ABI (AddressSanitizer + UBSan build).
 #include "pdpd_hip.h"
+#include "pdpd/hip_dpd_engine.hpp"
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -69,6 +70,57 @@ static void make(W &w, int Lx, int Ly, int Lz, double bonded, bool wall, double 
     d.bond_kappa = w.bk.data(); d.bond_r0 = w.br.data(); d.pt_angle_start = w.ptas.data();
     d.angle_head_bond = w.ahb.data(); d.angle_tail_bond = w.atb.data(); d.angle_kappa = w.ak.data(); d.angle_theta0 = w.at.data();
     d.n_polymers = p; d.polymer_type = w.ptype.data(); d.polymer_bead_start = w.pbs.data(); d.polymer_bead_ids = w.pbi.data();
+}
+
+// The same world as a WorldState (for the C++ engine classes).
+static WorldState to_state(const W &w)
+{
+    const pdpd_world_desc &d = w.d;
+    WorldState s;
+    s.box = {d.box[0], d.box[1], d.box[2]};
+    s.dt = d.dt; s.lambda = d.lambda; s.seed = d.seed; s.t = d.t;
+    const unsigned nt = d.n_bead_types;
+    for (unsigned i = 0; i < nt; i++) {
+        BeadType bt;
+        bt.name = "T" + std::to_string(i); bt.id = i; bt.stationary = d.stationary && d.stationary[i];
+        s.bead_types.push_back(bt);
+    }
+    s.interactions.resize(nt * nt);
+    for (unsigned k = 0; k < nt * nt; k++) {
+        s.interactions[k].conservative = d.conservative[k];
+        s.interactions[k].dissipative = d.dissipative[k];
+    }
+    for (unsigned t = 0; t < d.n_polymer_types; t++) {
+        PolymerType pt;
+        pt.name = "P" + std::to_string(t); pt.polymer_id = t;
+        for (unsigned b = d.pt_bond_start[t]; b < d.pt_bond_start[t + 1]; b++) {
+            Bond bd; bd.kappa = d.bond_kappa[b]; bd.r0 = d.bond_r0[b];
+            bd.bead_offset_head = d.bond_head[b]; bd.bead_offset_tail = d.bond_tail[b];
+            pt.bonds.push_back(bd);
+        }
+        for (unsigned a = d.pt_angle_start[t]; a < d.pt_angle_start[t + 1]; a++) {
+            BondPair bp; bp.kappa = d.angle_kappa[a]; bp.theta0 = d.angle_theta0[a];
+            bp.bond_offset_head = d.angle_head_bond[a]; bp.bond_offset_tail = d.angle_tail_bond[a];
+            pt.bond_pairs.push_back(bp);
+        }
+        s.polymer_types.push_back(pt);
+    }
+    s.beads.resize(d.n_beads);
+    for (unsigned p = 0; p < d.n_polymers; p++) {
+        Polymer poly; poly.polymer_id = p; poly.polymer_type = d.polymer_type[p];
+        for (unsigned k = d.polymer_bead_start[p]; k < d.polymer_bead_start[p + 1]; k++) {
+            const unsigned i = d.polymer_bead_ids[k];
+            poly.bead_ids.push_back(i);
+            if (s.polymer_types[poly.polymer_type].bead_types.size() <= k - d.polymer_bead_start[p])
+                s.polymer_types[poly.polymer_type].bead_types.push_back(d.bead_type[i]);
+            Bead &b = s.beads[i];
+            b.bead_id = i; b.polymer_id = d.polymer_id[i]; b.polymer_offset = d.polymer_offset[i];
+            b.bead_type = d.bead_type[i]; b.polymer_type = (uint8_t)poly.polymer_type; b.is_monomer = d.is_monomer[i];
+            for (int c = 0; c < 3; c++) { b.x[c] = d.x[3 * i + c]; b.v[c] = d.v[3 * i + c]; b.f[c] = d.f[3 * i + c]; }
+        }
+        s.polymers.push_back(poly);
+    }
+    return s;
 }
 
 #define CHECK(e) do { int rc_ = (e); if (rc_) { fprintf(stderr, "%s -> %d: %s\n", #e, rc_, pdpd_last_error()); exit(1); } } while (0)
@@ -169,6 +221,37 @@ int main(int argc, char **argv)
         for (double q : f) fs += std::fabs(q);
         fprintf(stderr, "  ok t=%lld sum|f|=%g\n", (long long)t, fs);
         CHECK(pdpd_ctx_destroy(ctx));
+    }
+    {   // the C++ batched engine over three contexts against each world run alone
+        fprintf(stderr, "case batch_engine\n");
+        W a, b, c, e;
+        make(a, 4, 5, 6, 0.2, false, 0.0, 21);
+        make(b, 5, 4, 4, 0.1, true, 0.5, 22);
+        make(c, 6, 5, 5, 0.0, false, 0.0, 23);
+        make(e, 4, 4, 4, 0.2, false, 0.0, 24);
+        W *ws[4] = {&a, &b, &c, &e};
+        std::vector<WorldState> st, solo;
+        for (W *w : ws) { st.push_back(to_state(*w)); solo.push_back(to_state(*w)); }
+        HipBatchDPDEngine batch(0, 3);
+        std::vector<int> slots;
+        for (auto &s : st) slots.push_back(batch.Attach(&s));
+        batch.SetSteps(slots[1], 2);
+        batch.Run(5);
+        for (size_t i = 0; i < solo.size(); i++) {
+            HipDPDEngine one;
+            one.Attach(&solo[i]);
+            one.Run(i == 1 ? 2 : 5);
+            for (size_t k = 0; k < solo[i].beads.size(); k++)
+                for (int q = 0; q < 3; q++)
+                    if (solo[i].beads[k].x[q] != st[i].beads[k].x[q] || solo[i].beads[k].v[q] != st[i].beads[k].v[q] ||
+                        solo[i].beads[k].f[q] != st[i].beads[k].f[q]) {
+                        fprintf(stderr, "batch engine: world %zu bead %zu differs from its solo run\n", i, k);
+                        return 1;
+                    }
+            if (solo[i].t != st[i].t) { fprintf(stderr, "batch engine: t differs\n"); return 1; }
+            one.Attach(nullptr);
+        }
+        fprintf(stderr, "  ok %zu worlds over %zu contexts\n", batch.size(), batch.parts());
     }
     fprintf(stderr, "all cases done\n");
     return 0;

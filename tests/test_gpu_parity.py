@@ -364,3 +364,26 @@ def test_interval_snapshot_overlaps_later_steps(gpu_lib):
     with pytest.raises(capi.PdpdError, match="no snapshot"):
         c.snapshot_download(s1)
     c.close()
+
+
+def test_ensemble_over_contexts_bitwise_equal_to_solo(gpu_lib):
+    """capi.Ensemble spreads worlds over contexts with their own streams; every
+    world's result equals its solo run bit for bit, with step budgets too."""
+    rng = np.random.default_rng(8)
+    worlds = [W.make_world(tuple(int(v) for v in rng.integers(4, 12, 3)), bonded_fraction=0.15, seed=90 + i,
+                           a_matrix=W.random_a_matrix(rng)) for i in range(7)]
+    _say("ensemble of 7 worlds over 3 contexts, 17 steps, one budget of 9")
+    ens = capi.Ensemble(0, parts=3)
+    slots = [ens.add(w) for w in worlds]
+    ens.set_steps(slots[2], 9)
+    ens.run(17)
+    ens.sync()
+    got = [ens.download(s) for s in slots]
+    ens.close()
+    for i, w in enumerate(worlds):
+        c, s = run_gpu(w, 9 if i == 2 else 17)
+        ref = c.download(s)
+        c.close()
+        assert got[i][3] == ref[3]
+        for a, b in zip(got[i][:3], ref[:3]):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
