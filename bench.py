@@ -136,7 +136,7 @@ def run_served(args, capi, W):
     arrivals join the running launches at interval boundaries."""
     reqs = W.served_requests(args.serve, seed=1)
     srv = capi.Server(device=0, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), interval_steps=500,
-                      time_scale=1.0)
+                      time_scale=1.0, contexts=args.serve_contexts)
     seen = [0]
 
     def on_interval(rid, done, t, x, v, f):
@@ -153,7 +153,7 @@ def run_served(args, capi, W):
             "p99_latency_s": st["p99_latency_s"], "mean_latency_s": st["mean_latency_s"],
             "value": st["bead_steps_per_s"], "unit": "bead-steps/s", "makespan_s": st["makespan_s"],
             "max_resident": st["max_resident"], "callbacks": seen[0], "interval_steps": 500,
-            "arrival_rate_per_s": 50.0, "hbm_budget_gib": args.serve_budget_gib}
+            "arrival_rate_per_s": 50.0, "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
 
 
 def copy_calibration(mib):
@@ -188,6 +188,7 @@ def main():
                     help="batched ensemble with the default launch shape instead of the autotuner's")
     ap.add_argument("--serve", type=int, default=64, help="served sweep with this many requests (0 = skip)")
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
+    ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
     ap.add_argument("--side", type=int, default=32)
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")

@@ -28,7 +28,9 @@
 extern "C" {
 #endif
 
-#define PDPD_ABI_VERSION 2
+/* 2: pdpd_world_desc gained cell_conservative / cell_dissipative;
+ * 3: pdpd_server_config gained contexts. */
+#define PDPD_ABI_VERSION 3
 
 enum {
     PDPD_OK = 0,
@@ -259,6 +261,8 @@ typedef struct pdpd_server_config {
     uint32_t interval_steps;      /* steps between admissions / callbacks (e.g. 500)     */
     uint32_t max_resident;        /* 0 = unlimited                                       */
     double time_scale;            /* arrival times are multiplied by this; 0 = all at once */
+    uint32_t contexts;            /* resident requests are spread over this many contexts
+                                     (streams whose launches overlap); 0 or 1 = one     */
 } pdpd_server_config;
 
 /* Called after every interval for every resident request with its exported

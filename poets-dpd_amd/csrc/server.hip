@@ -89,6 +89,7 @@ struct Request {
     std::unique_ptr<OwnedDesc> desc;
     double arrival;         // seconds after the start of the run (scaled)
     uint32_t steps;
+    uint32_t part = 0;      // the context the request is resident in
     uint32_t done = 0;
     uint32_t inflight = 0;  // steps enqueued whose results are not yet processed
     int slot = -1;
@@ -112,7 +113,8 @@ uint64_t device_bytes(const pdpd_world_desc &d)
 
 struct pdpd_server {
     pdpd_server_config cfg{};
-    pdpd_ctx *ctx = nullptr;
+    std::vector<pdpd_ctx *> ctxs;      // resident requests are spread over these (one stream each)
+    std::vector<uint64_t> part_beads;  // resident beads per context
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request *> waiting;
     std::vector<Request *> resident;
@@ -123,7 +125,7 @@ struct pdpd_server {
 
     ~pdpd_server()
     {
-        if (ctx) pdpd_ctx_destroy(ctx);
+        for (pdpd_ctx *c : ctxs) pdpd_ctx_destroy(c);
     }
 };
 
@@ -135,8 +137,15 @@ int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out)
     if (cfg->interval_steps == 0) return sfail(PDPD_ERR_INVALID, "interval_steps must be positive");
     std::unique_ptr<pdpd_server> s(new pdpd_server());
     s->cfg = *cfg;
-    int rc = pdpd_ctx_create(cfg->device, &s->ctx);
-    if (rc) return sfail(rc, pdpd_last_error());
+    const uint32_t parts = std::max<uint32_t>(1, cfg->contexts);
+    if (parts > 64) return sfail(PDPD_ERR_INVALID, "contexts must be at most 64");
+    for (uint32_t i = 0; i < parts; i++) {
+        pdpd_ctx *c = nullptr;
+        int rc = pdpd_ctx_create(cfg->device, &c);
+        if (rc) return sfail(rc, pdpd_last_error());
+        s->ctxs.push_back(c);
+    }
+    s->part_beads.assign(parts, 0);
     *out = s.release();
     return PDPD_OK;
 }
@@ -150,8 +159,11 @@ int pdpd_server_destroy(pdpd_server *s)
 int pdpd_server_set_config(pdpd_server *s, const char *key, const char *value)
 {
     if (!s) return sfail(PDPD_ERR_INVALID, "null server");
-    int rc = pdpd_set_config(s->ctx, key, value);
-    return rc ? sfail(rc, pdpd_last_error()) : PDPD_OK;
+    for (pdpd_ctx *c : s->ctxs) {
+        int rc = pdpd_set_config(c, key, value);
+        if (rc) return sfail(rc, pdpd_last_error());
+    }
+    return PDPD_OK;
 }
 
 int pdpd_server_set_callback(pdpd_server *s, pdpd_interval_fn fn, void *user)
@@ -224,14 +236,16 @@ int pdpd_server_run(pdpd_server *s)
                 const size_t n = r->res.n_beads;
                 r->x.resize(3 * n); r->v.resize(3 * n); r->f.resize(3 * n);
                 int64_t t = 0;
-                int rc = pdpd_world_snapshot_download(s->ctx, r->slot, r->x.data(), r->v.data(), r->f.data(), &t);
+                int rc = pdpd_world_snapshot_download(s->ctxs[r->part], r->slot, r->x.data(), r->v.data(),
+                                                      r->f.data(), &t);
                 if (rc) return rc;
                 if (s->cb) cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
                                           r->f.data()) == 0;
             }
             if (finished || cancel) {
-                int rc = pdpd_world_remove(s->ctx, r->slot);
+                int rc = pdpd_world_remove(s->ctxs[r->part], r->slot);
                 if (rc) return rc;
+                s->part_beads[r->part] -= r->res.n_beads;
                 r->slot = -1;
                 r->inflight = 0;
                 r->res.done_s = rel();
@@ -257,8 +271,12 @@ int pdpd_server_run(pdpd_server *s)
             if (r->arrival * scale > now) break;
             if (budget && used + r->bytes > budget && !s->resident.empty()) break;
             if (s->cfg.max_resident && s->resident.size() >= s->cfg.max_resident) break;
-            int rc = pdpd_world_add(s->ctx, &r->desc->d, &r->slot);
+            const uint32_t part = (uint32_t)(std::min_element(s->part_beads.begin(), s->part_beads.end()) -
+                                             s->part_beads.begin());
+            int rc = pdpd_world_add(s->ctxs[part], &r->desc->d, &r->slot);
             if (rc) return fail_rc(rc);
+            r->part = part;
+            s->part_beads[part] += r->res.n_beads;
             r->res.admit_s = now;
             r->res.status = 2;  // resident
             used += r->bytes;
@@ -281,12 +299,15 @@ int pdpd_server_run(pdpd_server *s)
         if (k) {
             for (Request *r : s->resident) {
                 const uint32_t adv = std::min(k, r->steps - r->done - r->inflight);
-                int rc = pdpd_world_set_steps(s->ctx, r->slot, adv);
+                int rc = pdpd_world_set_steps(s->ctxs[r->part], r->slot, adv);
                 if (rc) return fail_rc(rc);
                 if (adv) next.push_back({r, adv});
             }
-            int rc = pdpd_run(s->ctx, k);
-            if (rc) return fail_rc(rc);
+            for (uint32_t p = 0; p < s->ctxs.size(); p++) {
+                if (!s->part_beads[p]) continue;
+                int rc = pdpd_run(s->ctxs[p], k);
+                if (rc) return fail_rc(rc);
+            }
             for (auto &pa : next) pa.first->inflight += pa.second;
             s->stats.intervals++;
         }
@@ -299,13 +320,18 @@ int pdpd_server_run(pdpd_server *s)
             for (auto &pa : next)
                 if (pa.first->slot >= 0) pending.push_back(pa);
             // cancelled requests left the context; their interval is discarded
-            int rc = pdpd_snapshot(s->ctx);
-            if (rc) return fail_rc(rc);
+            for (uint32_t p = 0; p < s->ctxs.size(); p++) {
+                if (!s->part_beads[p]) continue;
+                int rc = pdpd_snapshot(s->ctxs[p]);
+                if (rc) return fail_rc(rc);
+            }
             have_pending = !pending.empty();
         }
     }
-    int rc = pdpd_sync(s->ctx);
-    if (rc) return fail_rc(rc);
+    for (pdpd_ctx *c : s->ctxs) {
+        int rc = pdpd_sync(c);
+        if (rc) return fail_rc(rc);
+    }
     s->stats.makespan_s = rel();
     s->stats.gpu_busy_s += s->stats.makespan_s - idle;
     s->stats.requests = s->reqs.size();

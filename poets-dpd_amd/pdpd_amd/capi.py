@@ -23,7 +23,7 @@ class PdpdError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 2          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
+ABI_VERSION = 3          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
 
 
 class WorldDesc(C.Structure):
@@ -66,7 +66,7 @@ class Stats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_uint64), ("interval_steps", C.c_uint32),
-                ("max_resident", C.c_uint32), ("time_scale", C.c_double)]
+                ("max_resident", C.c_uint32), ("time_scale", C.c_double), ("contexts", C.c_uint32)]
 
 
 class RequestResult(C.Structure):
@@ -472,9 +472,10 @@ class Ensemble:
 class Server:
     """Continuous-batching serving loop (pdpd_server_* in the C ABI)."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, interval_steps=500, max_resident=0, time_scale=1.0):
+    def __init__(self, device=0, hbm_budget_bytes=0, interval_steps=500, max_resident=0, time_scale=1.0, contexts=1):
         self.lib = load()
-        cfg = ServerConfig(device, int(hbm_budget_bytes), int(interval_steps), int(max_resident), float(time_scale))
+        cfg = ServerConfig(device, int(hbm_budget_bytes), int(interval_steps), int(max_resident), float(time_scale),
+                           int(contexts))
         h = C.c_void_p()
         check(self.lib.pdpd_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -33,11 +33,12 @@ def _solo(world, steps):
     return out
 
 
-def test_served_results_equal_solo_runs(gpu_lib):
+@pytest.mark.parametrize("contexts", [1, 3])
+def test_served_results_equal_solo_runs(gpu_lib, contexts):
     reqs = _requests(10)
     # a budget that holds only part of the requests at once forces queueing
     budget = sum(112 * w.n_beads + 8 * int(np.prod(w.box)) + 4096 for w, _, _ in reqs) // 3
-    srv = capi.Server(device=0, hbm_budget_bytes=budget, interval_steps=25, time_scale=0.0)
+    srv = capi.Server(device=0, hbm_budget_bytes=budget, interval_steps=25, time_scale=0.0, contexts=contexts)
     ids = [srv.submit(w, arr, steps) for w, steps, arr in reqs]
     print(f"serving {len(ids)} requests, budget {budget} B", flush=True)
     srv.run()

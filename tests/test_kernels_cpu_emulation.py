@@ -127,7 +127,7 @@ reqs = [(W.make_world((4, 4, 5), bonded_fraction=0.2, seed=600 + i), int(s)) for
 def solo(w, steps):
     c = capi.Context(0); s = c.add(w); c.run(steps); out = c.download(s); c.close(); return out
 budget = sum(112 * w.n_beads + 8 * int(np.prod(w.box)) + 4096 for w, _ in reqs) // 2
-srv = capi.Server(device=0, hbm_budget_bytes=budget, interval_steps=10, time_scale=0.0)
+srv = capi.Server(device=0, hbm_budget_bytes=budget, interval_steps=10, time_scale=0.0, contexts=2)
 seen = {}
 def cb(rid, done, t, x, v, f):
     seen.setdefault(rid, []).append((done, t))
