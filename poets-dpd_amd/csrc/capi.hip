@@ -89,6 +89,10 @@ struct WorldHost {
     std::vector<float4> up_pos, up_vel, up_frc;   // pending upload (bead order)
     std::vector<TermRec> bonds, angles;            // local bead indices, reference order
     std::vector<uint32_t> stationary_beads;        // local indices of beads of stationary types
+    // per bead (local index) its terms, in the order the per-polymer loop
+    // applies them (its bonds, then its angles): role << kRoleShift | term
+    // index within bonds then angles; built once, when the world is added
+    std::vector<uint32_t> term_start_local, term_list_local;
     uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t term_base = 0;                        // index of the world's first bonded term (bonds, then angles)
     int64_t t_at_layout = 0;                       // t when the current layout was built
@@ -402,7 +406,8 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_upload(ctx->fblocks, fb));
 
     // bonded terms with global bead indices; per bead (global index) its terms
-    // in the order the per-polymer loop applies them: its bonds, then its angles
+    // in the order the per-polymer loop applies them: its bonds, then its
+    // angles (each world's lists are built once, when it is added)
     std::vector<TermRec> trec;
     std::vector<uint32_t> tstart(nb + 1, 0), terms;
     if (ws.size() >= (1u << (32 - kTermWorldShift))) return fail(PDPD_ERR_INVALID, "too many worlds in one context");
@@ -411,24 +416,22 @@ int relayout(pdpd_ctx *ctx)
         wm.term_base = (uint32_t)trec.size();
         wm.t_at_layout = wm.t;
         const WorldHost &w = *ws[i];
+        const uint32_t base = (uint32_t)trec.size();
+        const uint32_t tstart0 = (uint32_t)terms.size();
+        for (uint32_t b = 0; b < w.n; b++) tstart[w.bead_off + b] = tstart0 + (w.term_start_local.empty() ? 0 : w.term_start_local[b]);
         if (w.bonds.empty() && w.angles.empty()) continue;
-        std::vector<std::vector<uint32_t>> per(w.n);
-        auto add = [&](const TermRec &r0, std::initializer_list<std::pair<uint32_t, uint32_t>> roles) {
-            const uint32_t k = (uint32_t)trec.size();
-            for (const auto &rb : roles) per[rb.second].push_back((rb.first << kRoleShift) | k);
-            TermRec r = r0;
-            r.a += w.bead_off; r.b += w.bead_off; r.c += w.bead_off;
-            r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
-            trec.push_back(r);
-        };
-        for (const TermRec &r : w.bonds) add(r, {{kRoleBondHead, r.a}, {kRoleBondTail, r.b}});
-        for (const TermRec &r : w.angles) add(r, {{kRoleAngleHead, r.a}, {kRoleAngleTail, r.c}, {kRoleAngleMid, r.b}});
-        if (trec.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
-        for (uint32_t b = 0; b < w.n; b++) tstart[w.bead_off + b + 1] = (uint32_t)per[b].size();
-        for (uint32_t b = 0; b < w.n; b++) terms.insert(terms.end(), per[b].begin(), per[b].end());
+        if (base + w.bonds.size() + w.angles.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
+        for (const auto *list : {&w.bonds, &w.angles})
+            for (const TermRec &r0 : *list) {
+                TermRec r = r0;
+                r.a += w.bead_off; r.b += w.bead_off; r.c += w.bead_off;
+                r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
+                trec.push_back(r);
+            }
+        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
+        for (uint32_t e : w.term_list_local) terms.push_back((e & ~kIndexMask) | ((e & kIndexMask) + base));
     }
-    // counts -> starts (beads of worlds without terms count 0); terms were appended in bead order
-    for (size_t k = 0; k < nb; k++) tstart[k + 1] += tstart[k];
+    tstart[nb] = (uint32_t)terms.size();
     HIP_TRY(dev_upload(ctx->terms, trec));
     HIP_TRY(dev_upload(ctx->term_start, tstart));
     HIP_TRY(dev_upload(ctx->bead_terms, terms));
@@ -800,6 +803,22 @@ int pdpd_world_add(pdpd_ctx *ctx, const pdpd_world_desc *d, int *slot_out)
     for (const auto &a : w.angles)
         if (d->is_monomer[a.a] || d->is_monomer[a.b] || d->is_monomer[a.c])
             return fail(PDPD_ERR_INVALID, "a bead in an angle bond is marked as a monomer");
+    // per bead its terms in the order they apply (bonds, then angles), by counting
+    if (!w.bonds.empty() || !w.angles.empty()) {
+        auto roles = [&](auto fn) {
+            uint32_t k = 0;
+            for (const TermRec &r : w.bonds) { fn(r.a, kRoleBondHead, k); fn(r.b, kRoleBondTail, k); k++; }
+            for (const TermRec &r : w.angles) {
+                fn(r.a, kRoleAngleHead, k); fn(r.c, kRoleAngleTail, k); fn(r.b, kRoleAngleMid, k); k++;
+            }
+        };
+        w.term_start_local.assign(w.n + 1, 0);
+        roles([&](uint32_t bead, uint32_t, uint32_t) { w.term_start_local[bead + 1]++; });
+        for (uint32_t b = 0; b < w.n; b++) w.term_start_local[b + 1] += w.term_start_local[b];
+        w.term_list_local.resize(w.term_start_local[w.n]);
+        std::vector<uint32_t> at(w.term_start_local.begin(), w.term_start_local.end() - 1);
+        roles([&](uint32_t bead, uint32_t role, uint32_t k) { w.term_list_local[at[bead]++] = (role << kRoleShift) | k; });
+    }
 
     int slot = ctx->next_slot++;
     w.slot = slot;
