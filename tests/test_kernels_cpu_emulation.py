@@ -146,14 +146,28 @@ for rid, (w, steps) in zip(ids[:3], reqs[:3]):
 ok &= srv.result(ids[3])["status"] == 3 and [d for d, _ in seen[ids[3]]] == [10, 20]
 print("serving", "ok" if ok else "WRONG", st, flush=True)
 srv.close()
+# an ensemble over three contexts: each world as if run alone, budgets too
+ens = capi.Ensemble(0, parts=3)
+es = [ens.add(w) for w, _ in reqs]
+ens.set_steps(es[1], 12)
+ens.run(20)
+ens.sync()
+ok = True
+for k, (w, _) in enumerate(reqs):
+    x, v, f, t = solo(w, 12 if k == 1 else 20)
+    gx, gv, gf, gt = ens.download(es[k])
+    ok &= gt == t and np.array_equal(gx, x) and np.array_equal(gv, v) and np.array_equal(gf, f)
+print("ensemble", "ok" if ok else "WRONG", flush=True)
+ens.close()
 """
 
 
 def test_emulated_serving_pipeline(emu_build):
     """The serving loop with one interval in flight (snapshot, next interval,
-    then downloads and callbacks): queueing under a memory budget, a callback
-    per interval, a cancellation, and results equal to solo runs bit for bit."""
+    then downloads and callbacks) over two contexts: queueing under a memory
+    budget, a callback per interval, a cancellation, and results equal to solo
+    runs bit for bit; an ensemble of worlds over three contexts likewise."""
     code = SERVE_PROBE % dict(pkg=os.path.join(ROOT, "poets-dpd_amd"), lib=os.path.join(emu_build, "libpdpd_emu.so"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
-    assert "serving ok" in r.stdout, r.stdout
+    assert "serving ok" in r.stdout and "ensemble ok" in r.stdout, r.stdout
