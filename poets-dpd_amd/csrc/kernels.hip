@@ -716,9 +716,13 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
         const BeadBlock blk = S.force_blocks[xcd_chunk(li, n_blocks)];
         if (S.ctr[blk.world].remaining == 0) continue;       // uniform: frozen world (never in a chain)
         const WorldDev *__restrict__ W = &S.worlds[blk.world];
-        for (int i = tid; i < kMaxTypes * kMaxTypes; i += kThreads) {
-            s_a[i] = W->a[i];
-            s_sg[i] = W->sg[i];
+        {   // the world's n_types^2 strengths (rows of kMaxTypes)
+            const int nt = (int)W->n_types;
+            for (int i = tid; i < nt * nt; i += kThreads) {
+                const int k = (i / nt) * kMaxTypes + i % nt;
+                s_a[k] = W->a[k];
+                s_sg[k] = W->sg[k];
+            }
         }
         const uint32_t s = blk.begin + lane;
         const bool live = s < blk.end;
