@@ -37,11 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(name="pair_force_polymer32_pmc.md"):
     """HBM bytes per pair-force launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when no such summary is present."""
-    path = os.path.join(ROOT, "profiles", "pair_force_polymer32_pmc.md")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -190,6 +190,10 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
     ap.add_argument("--side", type=int, default=32)
+    ap.add_argument("--force-band-rows", type=int, default=None,
+                    help="force-block band height for worlds with large y-z planes (0 = layout order)")
+    ap.add_argument("--large", action="store_true",
+                    help="plain engine on the large world (128^3 cells, 6.3M beads, 5%% bonded) instead")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
     ap.add_argument("--fused-integrate", type=int, default=None, choices=(0, 1, 2),
@@ -211,8 +215,13 @@ def main():
     from pdpd_amd import world as W
     capi.load()
 
-    world = W.config_polymer(seed=1, L=args.side)
+    if args.large:
+        args.side = 128
+        world = W.config_large(seed=1, L=args.side)
+    else:
+        world = W.config_polymer(seed=1, L=args.side)
     n = world.n_beads
+    bonded_pct = 5 if args.large else 10
     log(f"bench: polymer world {args.side}^3, {n} beads; warmup {args.warmup}, timed {args.steps}")
     ctx = capi.Context(0)
     slot = ctx.add(world)
@@ -220,6 +229,8 @@ def main():
         ctx.set_config("graph_steps", str(args.graph_steps))
     if args.fused_integrate is not None:
         ctx.set_config("fused_integrate", str(args.fused_integrate))
+    if args.force_band_rows is not None:
+        ctx.set_config("force_band_rows", str(args.force_band_rows))
     if args.autotune:
         log("bench: autotune " + ctx.autotune(20))
     ctx.run(args.warmup)
@@ -238,7 +249,7 @@ def main():
     st = ctx.stats()
     step_ms = elapsed * 1e3 / args.steps          # whole step, timed region
     achieved = n * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles("pair_force_large128_pmc.md" if args.large else "pair_force_polymer32_pmc.md")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_force", "kernel_ms": force_ms,
@@ -254,7 +265,7 @@ def main():
            "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"plain engine, polymer world {args.side}x{args.side}x{args.side} cells, "
-                                  f"density 3, {n} beads, 10% in H2T4 chains (Hookean k=128 r0=0.5, angle k=20)",
+                                  f"density 3, {n} beads, {bonded_pct}% in H2T4 chains (Hookean k=128 r0=0.5, angle k=20)",
                       "beads": n, "dt": 0.01, "engine": f"HipDPDEngine (graph-captured steps, fused_integrate={fused})"},
            "roofline": roofline}
     if args.batch:

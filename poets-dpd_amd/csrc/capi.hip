@@ -61,6 +61,8 @@ constexpr uint32_t kMaxBox = 1024;
 // faster at 98k beads (46.1 against 47.5 us per step) and slower at 6.3M
 // (1.77 against 1.74 ms): on by default up to this many resident beads.
 constexpr uint32_t kFuseAutoBeads = 1u << 20;
+constexpr uint64_t kBandMinCells = 64 * 64;   // y-z plane size from which force blocks are listed in bands
+constexpr uint32_t kBandRows = 16;            // default band height (y rows)
 constexpr uint32_t kMaxPolymerLength = 127;
 
 uint32_t bead_hash(uint32_t type, bool monomer, uint32_t polymer_id, uint32_t offset)
@@ -187,6 +189,7 @@ struct pdpd_ctx {
     uint32_t force_variant = 0;
     uint32_t lookback_tiles = 256;
     uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
+    uint32_t band_rows = kBandRows;       // force-block order of large worlds (0 = layout order)
 
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
 
@@ -257,7 +260,16 @@ namespace {
 
 // Work lists over all resident worlds, in layout (world, then cell) order. The
 // pair-force kernel deals contiguous shares of this list to the XCDs.
-void build_blocks(const std::vector<const WorldHost *> &ws, std::vector<BeadBlock> &bb, std::vector<BeadBlock> &fb)
+// A world whose y-z plane holds at least kBandMinCells cells has its force
+// blocks listed in bands of band_rows y-rows instead: band by band, each band
+// plane by plane (x), within a plane in slot order. An XCD then works through
+// a few planes of one band at a time, so the stencil neighbours its
+// workgroups read stay in its L2, instead of streaming whole planes. A
+// block's cell is estimated from its middle slot at the world's mean density
+// (slots are in cell order). The order changes which XCD evaluates a block and
+// when, never a block's beads or any sum, so forces are bit-identical.
+void build_blocks(const std::vector<const WorldHost *> &ws, uint32_t band_rows, std::vector<BeadBlock> &bb,
+                  std::vector<BeadBlock> &fb)
 {
     bb.clear();
     fb.clear();
@@ -265,8 +277,20 @@ void build_blocks(const std::vector<const WorldHost *> &ws, std::vector<BeadBloc
         const WorldHost &w = *ws[i];
         for (uint32_t b = 0; b < w.n; b += kBeadThreads)
             bb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kBeadThreads), 0});
+        const size_t first = fb.size();
         for (uint32_t b = 0; b < w.n; b += kForceBeads)
             fb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kForceBeads), 0});
+        const uint64_t plane = (uint64_t)w.L[1] * (uint64_t)w.L[2];
+        if (band_rows == 0 || plane < kBandMinCells || (uint32_t)w.L[1] <= band_rows || w.n == 0) continue;
+        const double cells_per_slot = (double)w.n_cells() / (double)w.n;
+        auto band_plane = [&](const BeadBlock &b) {
+            const uint64_t mid = ((uint64_t)(b.begin - w.bead_off) + (uint64_t)(b.end - w.bead_off)) / 2;
+            const uint64_t c = std::min<uint64_t>((uint64_t)((double)mid * cells_per_slot), w.n_cells() - 1);
+            const uint64_t y = (c / (uint64_t)w.L[2]) % (uint64_t)w.L[1];
+            return std::make_pair(y / band_rows, c / plane);
+        };
+        std::stable_sort(fb.begin() + first, fb.end(),
+                         [&](const BeadBlock &a, const BeadBlock &b) { return band_plane(a) < band_plane(b); });
     }
 }
 
@@ -399,7 +423,7 @@ int relayout(pdpd_ctx *ctx)
     ctx->ctr_host.assign(2 * std::max<size_t>(1, order.size()), WorldCtr{0, 0});
 
     std::vector<BeadBlock> bb, fb;
-    build_blocks(ws, bb, fb);
+    build_blocks(ws, ctx->band_rows, bb, fb);
     ctx->n_bblocks = (uint32_t)bb.size();
     ctx->n_fblocks = (uint32_t)fb.size();
     HIP_TRY(dev_upload(ctx->bblocks, bb));
@@ -1042,6 +1066,22 @@ static int parse_u32(const std::string &v, uint32_t lo, uint32_t hi, uint32_t *o
     return 0;
 }
 
+// Re-lists the force blocks of the current layout in the order band_rows asks
+// for (a layout not yet built takes it when it is).
+static int rebuild_force_blocks(pdpd_ctx *ctx)
+{
+    if (ctx->dirty || ctx->n_fblocks == 0) return PDPD_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<const WorldHost *> ws;
+    for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
+    std::vector<BeadBlock> bb, fb;
+    build_blocks(ws, ctx->band_rows, bb, fb);
+    if ((uint32_t)fb.size() != ctx->n_fblocks) return fail(PDPD_ERR_STATE, "force block count changed");
+    HIP_TRY(hipMemcpy(ctx->fblocks.p, fb.data(), fb.size() * sizeof(BeadBlock), hipMemcpyHostToDevice));
+    return PDPD_OK;
+}
+
 int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
 {
     if (!ctx || !key || !value) return fail(PDPD_ERR_INVALID, "null argument");
@@ -1073,6 +1113,16 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         ctx->graph_version = ~0ull;
         return PDPD_OK;
     }
+    if (k == "force_band_rows") {
+        uint32_t n;
+        if (parse_u32(v, 0, 1u << 16, &n)) return fail(PDPD_ERR_INVALID, "force_band_rows must be an integer >= 0");
+        if (n != ctx->band_rows) {
+            ctx->band_rows = n;
+            const int rc = rebuild_force_blocks(ctx);
+            if (rc != PDPD_OK) return rc;
+        }
+        return PDPD_OK;
+    }
     if (k == "force_variant") {
         uint32_t n;
         if (parse_u32(v, 0, kForceVariants - 1, &n))
@@ -1093,6 +1143,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     else if (k == "force_variant") v = std::to_string(ctx->force_variant);
     else if (k == "scan_lookback_tiles") v = std::to_string(ctx->lookback_tiles);
     else if (k == "fused_integrate") v = std::to_string(ctx->fuse_mode);
+    else if (k == "force_band_rows") v = std::to_string(ctx->band_rows);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -1113,6 +1164,9 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "fused_integrate=" + std::to_string(ctx->fuse_mode) +
          "\tchained steps, the pair-force kernel integrating the next step: 0 = off, 1 = on, 2 = on up to " +
          std::to_string(kFuseAutoBeads) + " resident beads\n";
+    s += "force_band_rows=" + std::to_string(ctx->band_rows) +
+         "\tpair-force work order of worlds with at least " + std::to_string(kBandMinCells) +
+         " cells per y-z plane: bands of this many y rows, plane by plane (0 = layout order)\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }

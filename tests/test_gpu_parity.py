@@ -248,6 +248,29 @@ def test_launch_shapes_bitwise_identical(gpu_lib, cfg):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("bands", [(16,), (8,), (48,), (0, 16), (16, 0, 4)])
+def test_banded_force_order_bitwise_identical(gpu_lib, bands):
+    """Force blocks of a world with a large y-z plane (80 x 64 cells) listed in
+    y-bands: same bits as the layout order, also when the order changes
+    between runs on a resident world."""
+    w = W.make_world((6, 80, 64), bonded_fraction=0.05, seed=43, a_matrix=W.random_a_matrix(np.random.default_rng(5)))
+    _say(f"banded force order {bands}")
+    c1, s1 = run_gpu(w, 24, force_band_rows=0)
+    base = c1.download_f32(s1)
+    c1.close()
+    c2, s2 = run_gpu(w, 0, force_band_rows=bands[0])
+    for b in bands[1:]:
+        c2.run(24 // len(bands))
+        c2.set_config("force_band_rows", b)
+        assert c2.get_config("force_band_rows") == str(b)
+    c2.run(24 - (24 // len(bands)) * (len(bands) - 1))
+    c2.sync()
+    got = c2.download_f32(s2)
+    c2.close()
+    for a, b in zip(base[:3], got[:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_export_round_trip_and_momentum(gpu_lib):
     """Attach/Run/export: t advances by n, positions stay in the box, total
     momentum stays at its initial value to FP32 rounding (pair forces are
