@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <pthread.h>
 
 /* ------------------------------------------------------------------------ */
 /* RNG: include/dpd/core/hash.hpp:20-32 (riscv_mix64_m2), 61-65 (get_t_hash), */
@@ -56,6 +57,16 @@ double orc_rand_sym_f64(uint64_t t_hash, uint32_t a, uint32_t b)
     int32_t rs;
     memcpy(&rs, &ru, 4);
     return rs * ldexp(1.0, -32);
+}
+
+/* orc_default_hash for the pair-force loop (same arithmetic, inlined). */
+static inline float default_hash_inl(uint64_t t_hash, uint32_t a, uint32_t b)
+{
+    if (a > b) { uint32_t tmp = a; a = b; b = tmp; }
+    uint32_t ru = (uint32_t)orc_mix64_m2(((uint64_t)a | ((uint64_t)b << 32)) ^ t_hash);
+    int32_t rs;
+    memcpy(&rs, &ru, 4);
+    return (float)rs * 0.00000000023283064365386962890625f;
 }
 
 /* include/dpd/core/dpd_state.hpp:18-60: bbbb 1ppp.. (monomer) / bbbb 0ooo oooo pppp.. */
@@ -423,102 +434,180 @@ int orc_f32_cells(const orc_f32_world *w, int32_t *key, int32_t *order, int32_t 
     return rc;
 }
 
+/* Worker state of the pair-force pass: the world's beads gathered into cell
+ * order (positions, shifted velocities, hashes), so that each row of the
+ * stencil reads contiguous memory. Every bead's force is still the same
+ * sequence of FP32 operations in the same order as the single-threaded
+ * definition below; threads only split the beads between them. */
+typedef struct f32_pair_job {
+    const orc_f32_world *w;
+    uint64_t t_hash;
+    const int32_t *order, *cs;
+    const float *sx, *sw;          /* cell order, 3 per bead */
+    const uint32_t *sh;            /* cell order */
+    float *f_out, *fabs_sum;
+    float scale;
+    int begin, end;                /* beads [begin, end) of the cell order */
+} f32_pair_job;
+
+/* Pair force of bead i: one partial sum per (x-plane, y-row) of the stencil,
+ * summed over the row's cells in z order (z-1, z, z+1) and slot order; rows
+ * combined (r0 + r1) + r2 per plane, planes (p0 + p1) + p2.
+ * dpd_maths_core_half_step_raw.hpp:129-216 with the terms grouped as
+ * a wr + sgw (scale u - sgw rdotv). */
+static void f32_pair_force_bead(const f32_pair_job *J, int k0)
+{
+    const orc_f32_world *w = J->w;
+    const int nt = w->n_types;
+    float *f_out = J->f_out;
+    const int i = J->order[k0];
+    if (f32_stationary(w, i)) {
+        /* a stationary bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390) */
+        f_out[3 * i] = f_out[3 * i + 1] = f_out[3 * i + 2] = 0.0f;
+        if (J->fabs_sum) J->fabs_sum[i] = 0.0f;
+        return;
+    }
+    int hc[3];
+    cell_key_f32(w, i, hc);
+    const float hx = J->sx[3 * (size_t)k0], hy = J->sx[3 * (size_t)k0 + 1], hz = J->sx[3 * (size_t)k0 + 2];
+    const float hvx = J->sw[3 * (size_t)k0], hvy = J->sw[3 * (size_t)k0 + 1], hvz = J->sw[3 * (size_t)k0 + 2];
+    const uint32_t hh = J->sh[k0];
+    const int htype = (int)(hh >> 28);
+    float px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, pz[3] = {0, 0, 0}, fa = 0.0f;
+    for (int ddx = -1; ddx <= 1; ddx++) {
+    float rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0};
+    for (int ddy = -1; ddy <= 1; ddy++) {
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int ddz = -1; ddz <= 1; ddz++) {
+        int oc3[3];
+        float sh[3];
+        int dd[3] = {ddx, ddy, ddz};
+        for (int d = 0; d < 3; d++) {
+            int o = hc[d] + dd[d];
+            sh[d] = 0.0f;
+            if (o < 0) { o += w->L[d]; sh[d] = -(float)w->L[d]; }
+            else if (o >= w->L[d]) { o -= w->L[d]; sh[d] = (float)w->L[d]; }
+            oc3[d] = o;
+        }
+        int oc = (oc3[0] * w->L[1] + oc3[1]) * w->L[2] + oc3[2];
+        for (int k = J->cs[oc]; k < J->cs[oc + 1]; k++) {
+            if (k == k0) continue;
+            const float *xo = J->sx + 3 * (size_t)k, *wo = J->sw + 3 * (size_t)k;
+            float ox = xo[0] + sh[0];
+            float oy = xo[1] + sh[1];
+            float oz = xo[2] + sh[2];
+            float dx = hx - ox, dy = hy - oy, dz = hz - oz;
+            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (!(r2 < 1.0f && r2 >= ORC_MIN_R2)) continue;
+            float r = sqrtf(r2);
+            float inv_r = 1.0f / r;
+            float wr = 1.0f - r;
+            float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+            float dvx = hvx - wo[0], dvy = hvy - wo[1], dvz = hvz - wo[2];
+            float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
+            uint32_t oh = J->sh[k];
+            int ti = htype * nt + (int)(oh >> 28);
+            float a, sg;
+            if (w->cell_par) {
+                const int hcl = (hc[0] * w->L[1] + hc[1]) * w->L[2] + hc[2];
+                const float *ph = w->cell_par + 2 * ((size_t)hcl * nt * nt + ti);
+                const float *po = w->cell_par + 2 * ((size_t)oc * nt * nt + ti);
+                a = (ph[0] + po[0]) * 0.5f;
+                sg = sqrtf((ph[1] + po[1]) * 0.5f);
+            } else {
+                a = w->a[ti];
+                sg = w->sg[ti];
+            }
+            float sgw = sg * wr;
+            float u = default_hash_inl(J->t_hash, hh, oh);
+            float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, J->scale * u));
+            fx = fmaf(ex, F, fx);
+            fy = fmaf(ey, F, fy);
+            fz = fmaf(ez, F, fz);
+            fa += fabsf(F);
+        }
+    }
+    rx[ddy + 1] = fx;
+    ry[ddy + 1] = fy;
+    rz[ddy + 1] = fz;
+    }
+    px[ddx + 1] = (rx[0] + rx[1]) + rx[2];
+    py[ddx + 1] = (ry[0] + ry[1]) + ry[2];
+    pz[ddx + 1] = (rz[0] + rz[1]) + rz[2];
+    }
+    f_out[3 * i] = (px[0] + px[1]) + px[2];
+    f_out[3 * i + 1] = (py[0] + py[1]) + py[2];
+    f_out[3 * i + 2] = (pz[0] + pz[1]) + pz[2];
+    if (J->fabs_sum) J->fabs_sum[i] = fa;
+}
+
+static void *f32_pair_worker(void *arg)
+{
+    const f32_pair_job *J = (const f32_pair_job *)arg;
+    for (int k = J->begin; k < J->end; k++) f32_pair_force_bead(J, k);
+    return NULL;
+}
+
+static int g_orc_threads = 1;
+
+/* Threads used by the FP32 pair-force pass (results do not depend on it). */
+void orc_set_threads(int n) { g_orc_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+
 void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, float *fabs_sum)
 {
     const int n = w->n_beads;
     const int nc = w->L[0] * w->L[1] * w->L[2];
-    const int nt = w->n_types;
-    int32_t *key = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    int32_t *key = (int32_t *)malloc(sizeof(int32_t) * nn);
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * nn);
     int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * ((size_t)nc + 1));
+    float *sx = (float *)malloc(sizeof(float) * 3 * nn);
+    float *sw = (float *)malloc(sizeof(float) * 3 * nn);
+    uint32_t *shs = (uint32_t *)malloc(sizeof(uint32_t) * nn);
     orc_f32_cells(w, key, order, cs);
-    const float scale = (float)sqrt(24.0 / (double)w->dt);
-
-    for (int i = 0; i < n; i++) {
-        if (f32_stationary(w, i)) {
-            /* a stationary bead ends the step with f = 0 (naive_dpd_engine_half_merge.hpp:387-390) */
-            f_out[3 * i] = f_out[3 * i + 1] = f_out[3 * i + 2] = 0.0f;
-            if (fabs_sum) fabs_sum[i] = 0.0f;
-            continue;
+    for (int k = 0; k < n; k++) {
+        const int j = order[k];
+        memcpy(sx + 3 * (size_t)k, w->x + 3 * (size_t)j, 3 * sizeof(float));
+        memcpy(sw + 3 * (size_t)k, w->w + 3 * (size_t)j, 3 * sizeof(float));
+        shs[k] = w->hash[j];
+    }
+    f32_pair_job base;
+    base.w = w;
+    base.t_hash = t_hash;
+    base.order = order;
+    base.cs = cs;
+    base.sx = sx;
+    base.sw = sw;
+    base.sh = shs;
+    base.f_out = f_out;
+    base.fabs_sum = fabs_sum;
+    base.scale = (float)sqrt(24.0 / (double)w->dt);
+    int nth = g_orc_threads;
+    if (n < 4096 * nth) nth = n / 4096 > 0 ? n / 4096 : 1;
+    if (nth <= 1) {
+        base.begin = 0;
+        base.end = n;
+        f32_pair_worker(&base);
+    } else {
+        f32_pair_job *jobs = (f32_pair_job *)malloc(sizeof(f32_pair_job) * (size_t)nth);
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nth);
+        for (int k = 0; k < nth; k++) {
+            jobs[k] = base;
+            jobs[k].begin = (int)((int64_t)n * k / nth);
+            jobs[k].end = (int)((int64_t)n * (k + 1) / nth);
         }
-        int hc[3];
-        cell_key_f32(w, i, hc);
-        const float hx = w->x[3 * i], hy = w->x[3 * i + 1], hz = w->x[3 * i + 2];
-        const float hvx = w->w[3 * i], hvy = w->w[3 * i + 1], hvz = w->w[3 * i + 2];
-        const uint32_t hh = w->hash[i];
-        const int htype = (int)(hh >> 28);
-        /* one partial sum per (x-plane, y-row) of the stencil, summed over the row's
-         * cells in z order; rows combined (r0 + r1) + r2 per plane, planes (p0 + p1) + p2 */
-        float px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, pz[3] = {0, 0, 0}, fa = 0.0f;
-        for (int ddx = -1; ddx <= 1; ddx++) {
-        float rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0};
-        for (int ddy = -1; ddy <= 1; ddy++) {
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (int ddz = -1; ddz <= 1; ddz++) {
-            int oc3[3];
-            float sh[3];
-            int dd[3] = {ddx, ddy, ddz};
-            for (int d = 0; d < 3; d++) {
-                int o = hc[d] + dd[d];
-                sh[d] = 0.0f;
-                if (o < 0) { o += w->L[d]; sh[d] = -(float)w->L[d]; }
-                else if (o >= w->L[d]) { o -= w->L[d]; sh[d] = (float)w->L[d]; }
-                oc3[d] = o;
-            }
-            int oc = (oc3[0] * w->L[1] + oc3[1]) * w->L[2] + oc3[2];
-            for (int k = cs[oc]; k < cs[oc + 1]; k++) {
-                int j = order[k];
-                if (j == i) continue;
-                float ox = w->x[3 * j] + sh[0];
-                float oy = w->x[3 * j + 1] + sh[1];
-                float oz = w->x[3 * j + 2] + sh[2];
-                float dx = hx - ox, dy = hy - oy, dz = hz - oz;
-                float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (!(r2 < 1.0f && r2 >= ORC_MIN_R2)) continue;
-                float r = sqrtf(r2);
-                float inv_r = 1.0f / r;
-                float wr = 1.0f - r;
-                float ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
-                float dvx = hvx - w->w[3 * j], dvy = hvy - w->w[3 * j + 1], dvz = hvz - w->w[3 * j + 2];
-                float rdotv = fmaf(ez, dvz, fmaf(ey, dvy, ex * dvx));
-                uint32_t oh = w->hash[j];
-                int ti = htype * nt + (int)(oh >> 28);
-                float a, sg;
-                if (w->cell_par) {
-                    const int hcl = (hc[0] * w->L[1] + hc[1]) * w->L[2] + hc[2];
-                    const float *ph = w->cell_par + 2 * ((size_t)hcl * nt * nt + ti);
-                    const float *po = w->cell_par + 2 * ((size_t)oc * nt * nt + ti);
-                    a = (ph[0] + po[0]) * 0.5f;
-                    sg = sqrtf((ph[1] + po[1]) * 0.5f);
-                } else {
-                    a = w->a[ti];
-                    sg = w->sg[ti];
-                }
-                float sgw = sg * wr;
-                float u = orc_default_hash(t_hash, hh, oh);
-                float F = fmaf(a, wr, sgw * fmaf(-sgw, rdotv, scale * u));
-                fx = fmaf(ex, F, fx);
-                fy = fmaf(ey, F, fy);
-                fz = fmaf(ez, F, fz);
-                fa += fabsf(F);
-            }
-        }
-        rx[ddy + 1] = fx;
-        ry[ddy + 1] = fy;
-        rz[ddy + 1] = fz;
-        }
-        px[ddx + 1] = (rx[0] + rx[1]) + rx[2];
-        py[ddx + 1] = (ry[0] + ry[1]) + ry[2];
-        pz[ddx + 1] = (rz[0] + rz[1]) + rz[2];
-        }
-        f_out[3 * i] = (px[0] + px[1]) + px[2];
-        f_out[3 * i + 1] = (py[0] + py[1]) + py[2];
-        f_out[3 * i + 2] = (pz[0] + pz[1]) + pz[2];
-        if (fabs_sum) fabs_sum[i] = fa;
+        for (int k = 1; k < nth; k++) pthread_create(&th[k], NULL, f32_pair_worker, &jobs[k]);
+        f32_pair_worker(&jobs[0]);
+        for (int k = 1; k < nth; k++) pthread_join(th[k], NULL);
+        free(jobs);
+        free(th);
     }
     free(key);
     free(order);
     free(cs);
+    free(sx);
+    free(sw);
+    free(shs);
 }
 
 static double min_image_f64(double d, double len)

@@ -139,6 +139,8 @@ void orc_f32_import(int n, const double *x, const double *v, const double *f, co
 void orc_f32_integrate(orc_f32_world *w);                          /* drift + wrap, in place */
 int  orc_f32_cells(const orc_f32_world *w, int32_t *key, int32_t *order, int32_t *cell_start);
 void orc_f32_pair_forces(const orc_f32_world *w, uint64_t t_hash, float *f_out, float *fabs_sum);
+/* Threads the FP32 pair-force pass may use (default 1); results do not depend on it. */
+void orc_set_threads(int n);
 void orc_f32_bonds(const orc_f32_world *w, float *f_inout);
 
 #ifdef __cplusplus

@@ -50,6 +50,19 @@ def _p(a):
 _lib = None
 
 
+def default_threads():
+    """Threads for the FP32 pair-force pass: ORC_THREADS, else the CPUs this
+    process may run on, at most 16 (results do not depend on the count)."""
+    env = os.environ.get("ORC_THREADS")
+    if env:
+        return max(1, int(env))
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -76,6 +89,8 @@ def lib():
         L.orc_f32_bonds.argtypes = [C.POINTER(OrcF32World), C.c_void_p]
         L.orc_f32_import.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                                      C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads(default_threads())
         _lib = L
     return _lib
 

@@ -129,28 +129,42 @@ class World:
     # Flattened topology in the reference's iteration order: per polymer, the
     # bonds of its type then the bond pairs (naive_dpd_engine.hpp:567-582).
     def flat_topology(self):
-        bond_start = [0]
-        angle_start = [0]
-        bh, bt, bk, br = [], [], [], []
-        ah, am, at, ak, a0 = [], [], [], [], []
-        for p in range(len(self.polymer_type_of)):
-            pt = self.polymer_types[int(self.polymer_type_of[p])]
-            ids = self.polymer_bead_ids[self.polymer_bead_start[p]:self.polymer_bead_start[p + 1]]
-            for (h, t, k, r0) in pt.bonds:
-                bh.append(int(ids[h])); bt.append(int(ids[t])); bk.append(k); br.append(r0)
-            for (hb, tb, k, th) in pt.bond_pairs:
-                hbond = pt.bonds[hb]
-                tbond = pt.bonds[tb]
-                ah.append(int(ids[hbond[0]])); am.append(int(ids[hbond[1]])); at.append(int(ids[tbond[1]]))
-                ak.append(k); a0.append(th)
-            bond_start.append(len(bh))
-            angle_start.append(len(ah))
+        """Per polymer (in polymer order) the bonds of its type, then its bond
+        pairs as (head, middle, tail) bead indices, with per-polymer start
+        offsets. Built per polymer type with array indexing; the order is the
+        per-polymer loop's."""
+        n_poly = len(self.polymer_type_of)
+        ptype = np.asarray(self.polymer_type_of, dtype=np.int64)
+        pstart = np.asarray(self.polymer_bead_start, dtype=np.int64)
+        ids = np.asarray(self.polymer_bead_ids, dtype=np.int64)
+        nb_t = np.array([len(pt.bonds) for pt in self.polymer_types], dtype=np.int64)
+        na_t = np.array([len(pt.bond_pairs) for pt in self.polymer_types], dtype=np.int64)
+        nb_p = nb_t[ptype] if n_poly else np.zeros(0, np.int64)
+        na_p = na_t[ptype] if n_poly else np.zeros(0, np.int64)
+        bond_start = np.concatenate([[0], np.cumsum(nb_p)]).astype(np.int64)
+        angle_start = np.concatenate([[0], np.cumsum(na_p)]).astype(np.int64)
+        nb, na = int(bond_start[-1]), int(angle_start[-1])
+        bh = np.zeros(nb, np.int64); bt = np.zeros(nb, np.int64)
+        bk = np.zeros(nb, np.float64); br = np.zeros(nb, np.float64)
+        ah = np.zeros(na, np.int64); am = np.zeros(na, np.int64); at = np.zeros(na, np.int64)
+        ak = np.zeros(na, np.float64); a0 = np.zeros(na, np.float64)
+        for t, pt in enumerate(self.polymer_types):
+            polys = np.nonzero(ptype == t)[0]
+            if polys.size == 0:
+                continue
+            base = pstart[polys]
+            for j, (h, tl, k, r0) in enumerate(pt.bonds):
+                dst = bond_start[polys] + j
+                bh[dst] = ids[base + h]; bt[dst] = ids[base + tl]; bk[dst] = k; br[dst] = r0
+            for j, (hb, tb, k, th) in enumerate(pt.bond_pairs):
+                dst = angle_start[polys] + j
+                ah[dst] = ids[base + pt.bonds[hb][0]]; am[dst] = ids[base + pt.bonds[hb][1]]
+                at[dst] = ids[base + pt.bonds[tb][1]]; ak[dst] = k; a0[dst] = th
         i32 = lambda a: np.asarray(a, dtype=np.int32)
-        f64 = lambda a: np.asarray(a, dtype=np.float64)
         return dict(poly_bond_start=i32(bond_start), poly_angle_start=i32(angle_start),
-                    bond_head=i32(bh), bond_tail=i32(bt), bond_kappa=f64(bk), bond_r0=f64(br),
+                    bond_head=i32(bh), bond_tail=i32(bt), bond_kappa=bk, bond_r0=br,
                     angle_head=i32(ah), angle_mid=i32(am), angle_tail=i32(at),
-                    angle_kappa=f64(ak), angle_theta0=f64(a0))
+                    angle_kappa=ak, angle_theta0=a0)
 
 
 class WorldBuilder:

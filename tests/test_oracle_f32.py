@@ -103,3 +103,19 @@ def test_half_merge_port_matches_naive_physics():
         b = O.F64State(w).run_half_merge(2, nthreads=thr)
         assert np.abs(a.f - b.f).max() < 1e-4
         assert np.abs(a.x - b.x).max() < 1e-7
+
+
+def test_f32_pair_forces_independent_of_threads():
+    """The FP32 pair-force pass splits beads over threads; every bead's sum is
+    the same sequence of operations, so 1 and 7 threads agree bit for bit."""
+    w = W.make_world((24, 24, 20), bonded_fraction=0.1, seed=21, a_matrix=W.random_a_matrix(np.random.default_rng(8)))
+    out = []
+    try:
+        for thr in (1, 7):
+            O.lib().orc_set_threads(thr)
+            s = O.F32State(w).step(2)
+            out.append((s.x.copy(), s.w.copy(), s.f.copy(), s.fabs.copy()))
+    finally:
+        O.lib().orc_set_threads(O.default_threads())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
