@@ -1,20 +1,36 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X DPD engine on the polymer world (32x32x32 cells,
-density 3, 98,304 beads, 10% of beads in H2T4 chains), one GPU.
+"""Throughput of the MI355X DPD engine, one GPU. Prints ONE JSON line.
 
-bead-steps/s = beads x steps / wall time of the timed run, state resident in
-HBM (the reference's benchmark_engine.cpp:128-147 rule with Attach excluded).
-Prints ONE JSON line. Extra keys:
-  roofline      pair-force kernel: algorithmic bytes / its average launch duration
-                (HIP events on the engine's stream around back-to-back launches
-                of the last timed step's pair-force kernel)
+Headline (`value`): the batched engine on the 256-world mixed ensemble
+(sides drawn uniformly from 8..32 cells per axis, density 3, each world with
+its own seed, a_ij in [25, 80] and 0..20% of beads in H2T4 chains; about 6.0M
+beads), launch shape chosen by the engine's autotuner. bead-steps/s =
+sum of beads x timed steps / wall time of the timed run, state resident in
+HBM (the rule of the reference's benchmark_engine.cpp:128-147, Attach
+excluded). The warm-up is rounded up to whole captured step graphs so the
+timed steps replay a graph that already exists.
+
+Extra keys:
+  roofline      the pair-force kernel of the headline workload: algorithmic
+                bytes per launch / its average launch duration, HIP events on
+                the engine's stream around each kernel of chained steps (as the
+                step runs them); traffic = HBM bytes per launch from the
+                committed rocprofv3 --pmc summary of the same workload
+  batched       the same ensemble for 5000 steps, against the same 256 worlds
+                run one at a time through the plain engine for 5000 steps each
+  plain_polymer32 / plain_large128
+                the plain engine on the polymer world (32^3 cells, 98,304
+                beads, 10% bonded) and the large world (128^3 cells, 6.3M
+                beads, 5% bonded), with per-kernel times
+  served        served sweep: Poisson arrivals (50/s), cubic sides
+                log-uniform in [8, 48], 1k..20k steps, interval callbacks every
+                500 steps, an HBM budget that holds only part of the requests;
+                p50/p99 latency, admission wait and aggregate bead-steps/s
   cpu_baseline  the reference's NaiveDPDEngineHalfMerge compiled from its
-                sources (-O3 -ffast-math), one thread, bounded sample
-  cpu_port_mt   the repository's multi-threaded half-merge restatement
-  batched       256-world mixed-size ensemble (8..32 cells per side), one
-                launch set per step, vs a sample of the same worlds one at a time
-  served        served sweep: Poisson arrivals, continuous batching, p50/p99
-                latency and aggregate bead-steps/s
+                sources (-O3 -ffast-math), one thread, bounded sample of the
+                polymer world
+  cpu_port_mt   the repository's multi-threaded FP64 half-merge restatement on
+                every CPU this process may use, CPU model stated
 """
 import argparse
 import json
@@ -31,16 +47,16 @@ FP32_PEAK_TFLOPS = 157.3
 FORCE_BYTES_PER_BEAD = 40      # read x, v, key (28 B) + write f (12 B)
 STEP_BYTES_PER_BEAD = 80       # algorithmic bytes of a whole step per bead
 FLOP_PER_BEAD_STEP = 282       # 6.283 pairs x 41 + 24
+KERNELS = ("k_integrate_ms", "k_scan_ms", "k_scatter_ms", "k_reorder_ms", "k_force_ms", "k_stationary_ms")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(name="pair_force_polymer32_pmc.md"):
-    """HBM bytes per pair-force launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when no such summary is present."""
+def traffic_from_profiles(name):
+    """HBM bytes per pair-force launch from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -53,23 +69,40 @@ def traffic_from_profiles(name="pair_force_polymer32_pmc.md"):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def usable_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(world, target_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O   # the checker's tree, used here only to time the CPU baseline
     out = {}
+    model = cpu_model()
     if O.ref_available(fast=True):
         timing = {}
-        steps = 1
-        O.ref_run(world, "half_merge", steps, fast=True, timing=timing)
+        O.ref_run(world, "half_merge", 1, fast=True, timing=timing)
         per = max(timing["run_s"], 1e-6)
         steps = max(1, min(2000, int(target_s / per)))
         O.ref_run(world, "half_merge", steps, fast=True, timing=timing)
         out["cpu_baseline"] = {
             "value": world.n_beads * steps / timing["run_s"], "unit": "bead-steps/s", "cores": 1,
-            "kind": "reference",
+            "kind": "reference", "cpu": model,
             "sample": f"NaiveDPDEngineHalfMerge built from the reference sources (-O3 -ffast-math), "
-                      f"same polymer world, {steps} steps, Run timed alone"}
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+                      f"polymer world 32^3 ({world.n_beads} beads), {steps} steps, Run timed alone"}
+    nthreads = usable_cpus()
     st = O.F64State(world)
     t0 = time.perf_counter()
     st.run_half_merge(1, nthreads=nthreads)
@@ -80,7 +113,9 @@ def cpu_baseline(world, target_s):
     st.run_half_merge(steps, nthreads=nthreads)
     dt = time.perf_counter() - t0
     port = {"value": world.n_beads * steps / dt, "unit": "bead-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"multi-threaded FP64 half-merge restatement, same world, {steps} steps"}
+            "cpu": model, "cpus_on_host": os.cpu_count(),
+            "sample": f"multi-threaded FP64 half-merge restatement (8-colour waves, one thread per usable CPU), "
+                      f"polymer world 32^3, {steps} steps"}
     if "cpu_baseline" not in out:
         out["cpu_baseline"] = port
     else:
@@ -88,52 +123,99 @@ def cpu_baseline(world, target_s):
     return out
 
 
-def run_batched(args, capi, W):
-    """Mixed-size ensemble: all worlds advance together (one launch set per step
-    in each of the ensemble's contexts, whose streams overlap), vs the same
-    worlds one at a time through the plain engine."""
-    worlds = W.config_batch(n_worlds=args.batch, seed=1)
-    total = sum(w.n_beads for w in worlds)
-    steps = args.batch_steps
-    parts = args.batch_parts if args.batch_parts else capi.suggested_parts(total)
-    ctx = capi.Ensemble(0, parts)
-    for w in worlds:
-        ctx.add(w)
-    tune = ctx.autotune(10) if not args.no_batch_autotune else "heuristic"
-    ctx.run(2)
+def graph_steps_of(ctx):
+    try:
+        return max(0, int(ctx.get_config("graph_steps")))
+    except Exception:
+        return 0
+
+
+def warm(ctx, warmup):
+    """Warm-up steps, rounded up to whole step graphs (even graph length, so
+    the buffers end where they started) so the timed run replays a graph that
+    is already captured. Returns the steps actually run."""
+    g = graph_steps_of(ctx) & ~1
+    n = max(warmup, 1)
+    if g >= 2:
+        n = -(-n // g) * g
+    ctx.run(n)
     ctx.sync()
+    return n
+
+
+def timed(ctx, steps):
     t0 = time.perf_counter()
     ctx.run(steps)
     ctx.sync()
-    tb = time.perf_counter() - t0
-    batched = total * steps / tb
-    ctx.close()
-    # one at a time, a sample of the worlds, same steps
-    k = min(len(worlds), args.batch_solo_sample)
-    solo_time = 0.0
-    solo_beads = 0
-    for w in worlds[:k]:
+    return time.perf_counter() - t0
+
+
+def kernel_profile(contexts, steps):
+    """Per-kernel HIP-event times (ms per step) of chained steps, summed over
+    the contexts (each profiled on its own)."""
+    acc = {k: 0.0 for k in KERNELS}
+    for c in contexts:
+        if not c.worlds:
+            continue
+        c.run_profiled(steps)
+        st = c.stats()
+        for k in KERNELS:
+            acc[k] += st[k]
+    return acc
+
+
+def roofline_of(n_beads, prof, step_ms, traffic):
+    force_ms = prof["k_force_ms"]
+    achieved = n_beads * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_force", "kernel_ms": force_ms,
+            "bytes_per_launch": n_beads * FORCE_BYTES_PER_BEAD,
+            "valu_fp32_frac": n_beads * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
+            "step_hbm_frac": n_beads * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "kernel_ms_per_step": {k[2:-3]: prof[k] for k in KERNELS}}
+
+
+def make_ensemble(capi, worlds, parts):
+    total = sum(w.n_beads for w in worlds)
+    parts = parts if parts else capi.suggested_parts(total)
+    ens = capi.Ensemble(0, parts)
+    for w in worlds:
+        ens.add(w)
+    return ens, parts
+
+
+def solo_leg(capi, worlds, steps, warmup):
+    """The worlds one at a time through the plain engine, `steps` each, with
+    the plain engine's default launch shape; wall time of the timed runs."""
+    total_t = 0.0
+    bead_steps = 0
+    for w in worlds:
         c = capi.Context(0)
         c.add(w)
-        c.run(2)
-        c.sync()
-        t0 = time.perf_counter()
-        c.run(steps)
-        c.sync()
-        solo_time += time.perf_counter() - t0
-        solo_beads += w.n_beads
+        warm(c, warmup)
+        total_t += timed(c, steps)
+        bead_steps += w.n_beads * steps
         c.close()
-    solo = solo_beads * steps / solo_time
-    return {"worlds": len(worlds), "beads": total, "steps": steps, "value": batched, "unit": "bead-steps/s",
-            "one_at_a_time": solo, "one_at_a_time_sample": k, "speedup": batched / solo, "contexts": parts,
-            "tuning": tune}
+    return bead_steps / total_t, total_t
+
+
+def run_plain(capi, world, steps, warmup, profile_steps, traffic_name):
+    ctx = capi.Context(0)
+    ctx.add(world)
+    w_done = warm(ctx, warmup)
+    el = timed(ctx, steps)
+    prof = kernel_profile([ctx], profile_steps)
+    step_ms = el * 1e3 / steps
+    out = {"value": world.n_beads * steps / el, "unit": "bead-steps/s", "beads": world.n_beads, "steps": steps,
+           "warmup": w_done, "ms_per_step": step_ms,
+           "roofline": roofline_of(world.n_beads, prof, step_ms, traffic_from_profiles(traffic_name))}
+    ctx.close()
+    return out
 
 
 def run_served(args, capi, W):
-    """Served sweep: Poisson arrivals (50/s), cubic sides log-uniform in
-    [8, 48] cells, 1k-20k steps, per-request a_ij and bonded fraction, interval
-    callbacks every 500 steps, admission by an HBM budget. Continuous batching:
-    arrivals join the running launches at interval boundaries."""
+    """Served sweep with continuous batching: arrivals join the resident
+    requests at interval boundaries while the HBM budget allows."""
     reqs = W.served_requests(args.serve, seed=1)
     srv = capi.Server(device=0, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), interval_steps=500,
                       time_scale=1.0, contexts=args.serve_contexts)
@@ -143,17 +225,30 @@ def run_served(args, capi, W):
         seen[0] += 1
         return True
     srv.set_callback(on_interval, copy=False)
+    ids = []
     for r in reqs:
         w = W.make_world(r["L"], bonded_fraction=r["bonded_fraction"], seed=r["seed"], a_matrix=r["a_matrix"])
-        srv.submit(w, r["arrival"], r["steps"])
+        ids.append(srv.submit(w, r["arrival"], r["steps"]))
     srv.run()
     st = srv.stats()
+    waits = sorted(max(0.0, res["admit_s"] - res["arrival_s"])
+                   for res in (srv.result(i, with_state=False) for i in ids))
     srv.close()
+
+    def pct(a, p):
+        if not a:
+            return 0.0
+        pos = p * (len(a) - 1)
+        lo = int(pos)
+        hi = min(len(a) - 1, lo + 1)
+        return a[lo] + (a[hi] - a[lo]) * (pos - lo)
     return {"requests": len(reqs), "completed": st["completed"], "p50_latency_s": st["p50_latency_s"],
             "p99_latency_s": st["p99_latency_s"], "mean_latency_s": st["mean_latency_s"],
+            "p50_admission_wait_s": pct(waits, 0.5), "p99_admission_wait_s": pct(waits, 0.99),
             "value": st["bead_steps_per_s"], "unit": "bead-steps/s", "makespan_s": st["makespan_s"],
             "max_resident": st["max_resident"], "callbacks": seen[0], "interval_steps": 500,
-            "arrival_rate_per_s": 50.0, "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
+            "arrival_rate_per_s": 50.0, "last_arrival_s": reqs[-1]["arrival"] if reqs else 0.0,
+            "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
 
 
 def copy_calibration(mib):
@@ -172,39 +267,36 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--force-reps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of the headline ensemble")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=256, help="batched ensemble of this many worlds (0 = skip)")
-    ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--batch-solo-sample", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=256, help="worlds in the ensemble")
+    ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the batched vs one-at-a-time comparison "
+                                                                 "(0 = skip it)")
+    ap.add_argument("--batch-solo-sample", type=int, default=0,
+                    help="one-at-a-time leg over the first N worlds only (0 = all of them)")
     ap.add_argument("--batch-parts", type=int, default=0,
                     help="contexts (streams) the ensemble is spread over (0 = the engine's suggestion)")
-    ap.add_argument("--autotune", action="store_true", help="autotune the plain engine's launch shape too")
-    ap.add_argument("--no-batch-autotune", action="store_true",
-                    help="batched ensemble with the default launch shape instead of the autotuner's")
-    ap.add_argument("--serve", type=int, default=64, help="served sweep with this many requests (0 = skip)")
-    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the autotuner's")
+    ap.add_argument("--plain-steps", type=int, default=1000)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine polymer and large worlds")
+    ap.add_argument("--no-large", action="store_true", help="skip the plain-engine large world")
+    ap.add_argument("--serve", type=int, default=256, help="served sweep with this many requests (0 = skip)")
+    ap.add_argument("--serve-budget-gib", type=float, default=0.5)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
-    ap.add_argument("--side", type=int, default=32)
-    ap.add_argument("--force-band-rows", type=int, default=None,
-                    help="force-block band height for worlds with large y-z planes (0 = layout order)")
-    ap.add_argument("--large", action="store_true",
-                    help="plain engine on the large world (128^3 cells, 6.3M beads, 5%% bonded) instead")
+    ap.add_argument("--only", choices=("ensemble", "polymer32", "large128"), default=None,
+                    help="run one workload only (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
-    ap.add_argument("--fused-integrate", type=int, default=None, choices=(0, 1, 2),
-                    help="chained steps: 0 off, 1 on, 2 automatic (the engine's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB, 5 times (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent worlds shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(independent worlds shard across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
@@ -215,68 +307,77 @@ def main():
     from pdpd_amd import world as W
     capi.load()
 
-    if args.large:
-        args.side = 128
-        world = W.config_large(seed=1, L=args.side)
-    else:
-        world = W.config_polymer(seed=1, L=args.side)
-    n = world.n_beads
-    bonded_pct = 5 if args.large else 10
-    log(f"bench: polymer world {args.side}^3, {n} beads; warmup {args.warmup}, timed {args.steps}")
-    ctx = capi.Context(0)
-    slot = ctx.add(world)
-    if args.graph_steps is not None:
-        ctx.set_config("graph_steps", str(args.graph_steps))
-    if args.fused_integrate is not None:
-        ctx.set_config("fused_integrate", str(args.fused_integrate))
-    if args.force_band_rows is not None:
-        ctx.set_config("force_band_rows", str(args.force_band_rows))
-    if args.autotune:
-        log("bench: autotune " + ctx.autotune(20))
-    ctx.run(args.warmup)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.run(args.steps)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    value = n * args.steps / elapsed
-    log(f"bench: {value:.4e} bead-steps/s ({elapsed * 1e3 / args.steps:.4f} ms/step)")
+    if args.only in ("polymer32", "large128"):
+        world = W.config_polymer(seed=1) if args.only == "polymer32" else W.config_large(seed=1)
+        ctx = capi.Context(0)
+        ctx.add(world)
+        if args.graph_steps is not None:
+            ctx.set_config("graph_steps", str(args.graph_steps))
+        warm(ctx, args.warmup)
+        el = timed(ctx, args.steps)
+        log(f"bench: {args.only}: {world.n_beads * args.steps / el:.4e} bead-steps/s")
+        ctx.close()
+        return
 
-    # the dominant kernel's average launch duration: HIP events on the engine's
-    # stream around back-to-back launches of the last step's pair-force kernel
-    force_ms = ctx.time_force(args.force_reps)
-    ctx.run_profiled(args.profile_steps)
-    st = ctx.stats()
-    step_ms = elapsed * 1e3 / args.steps          # whole step, timed region
-    achieved = n * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles("pair_force_large128_pmc.md" if args.large else "pair_force_polymer32_pmc.md")
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "k_force", "kernel_ms": force_ms,
-                "valu_fp32_frac": n * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
-                "step_hbm_frac": n * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-                "kernel_ms_per_step": {k[2:-3]: st[k] for k in ("k_integrate_ms", "k_scan_ms", "k_scatter_ms",
-                                                               "k_reorder_ms", "k_force_ms", "k_stationary_ms")}}
-    x, v, f, t = ctx.download(slot)
-    fused = ctx.get_config("fused_integrate")
-    ctx.close()
+    # ---- headline: the 256-world ensemble, batched engine
+    worlds = W.config_batch(n_worlds=args.batch, seed=1)
+    total = sum(w.n_beads for w in worlds)
+    log(f"bench: ensemble of {len(worlds)} worlds, {total} beads")
+    ens, parts = make_ensemble(capi, worlds, args.batch_parts)
+    if args.graph_steps is not None:
+        ens.set_config("graph_steps", str(args.graph_steps))
+    tuning = "default launch shape" if args.no_autotune else ens.autotune(10)
+    log(f"bench: tuning: {tuning}")
+    if args.only == "ensemble":
+        warm(ens, args.warmup)
+        el = timed(ens, args.steps)
+        log(f"bench: ensemble: {total * args.steps / el:.4e} bead-steps/s")
+        ens.close()
+        return
+    w_done = warm(ens, args.warmup)
+    el = timed(ens, args.steps)
+    value = total * args.steps / el
+    step_ms = el * 1e3 / args.steps
+    log(f"bench: {value:.4e} bead-steps/s ({step_ms:.4f} ms/step)")
+    prof = kernel_profile(ens.parts, args.profile_steps)
+    roofline = roofline_of(total, prof, step_ms, traffic_from_profiles("pair_force_ensemble256_pmc.md"))
 
     out = {"metric": METRIC, "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"plain engine, polymer world {args.side}x{args.side}x{args.side} cells, "
-                                  f"density 3, {n} beads, {bonded_pct}% in H2T4 chains (Hookean k=128 r0=0.5, angle k=20)",
-                      "beads": n, "dt": 0.01, "engine": f"HipDPDEngine (graph-captured steps, fused_integrate={fused})"},
+           "warmup": w_done, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"batched engine, ensemble of {len(worlds)} worlds (sides uniform in 8..32 cells "
+                                  f"per axis, density 3, own seed and a_ij in [25,80], 0-20% of beads in H2T4 chains)",
+                      "worlds": len(worlds), "beads": total, "dt": 0.01, "contexts": parts, "tuning": tuning},
            "roofline": roofline}
-    if args.batch:
-        log(f"bench: batched ensemble of {args.batch} worlds")
-        out["batched"] = run_batched(args, capi, W)
+
+    if args.batch_steps:
+        log(f"bench: batched vs one at a time, {args.batch_steps} steps")
+        tb = timed(ens, args.batch_steps)
+        batched = total * args.batch_steps / tb
+        sample = worlds[:args.batch_solo_sample] if args.batch_solo_sample else worlds
+        solo, solo_t = solo_leg(capi, sample, args.batch_steps, args.warmup)
+        out["batched"] = {"worlds": len(worlds), "beads": total, "steps": args.batch_steps, "value": batched,
+                          "unit": "bead-steps/s", "seconds": tb, "one_at_a_time": solo,
+                          "one_at_a_time_seconds": solo_t, "one_at_a_time_sample": len(sample),
+                          "speedup": batched / solo, "contexts": parts, "tuning": tuning}
+        log(f"bench: batched {batched:.4e}, one at a time {solo:.4e} -> {batched / solo:.2f}x")
+    ens.close()
+
+    if not args.no_plain:
+        log("bench: plain engine, polymer world 32^3")
+        out["plain_polymer32"] = run_plain(capi, W.config_polymer(seed=1), args.plain_steps, args.warmup,
+                                           args.profile_steps, "pair_force_polymer32_pmc.md")
+        if not args.no_large:
+            log("bench: plain engine, large world 128^3")
+            out["plain_large128"] = run_plain(capi, W.config_large(seed=1), max(20, args.plain_steps // 10),
+                                              args.warmup, max(4, args.profile_steps // 4),
+                                              "pair_force_large128_pmc.md")
     if args.serve:
         log(f"bench: served sweep of {args.serve} requests")
         out["served"] = run_served(args, capi, W)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
-        out.update(cpu_baseline(world, args.cpu_seconds))
+        out.update(cpu_baseline(W.config_polymer(seed=1), args.cpu_seconds))
     print(json.dumps(out), flush=True)
 
 

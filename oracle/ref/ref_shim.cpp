@@ -1,7 +1,10 @@
 // ref_shim.cpp -- C ABI around the reference's own engines, compiled from the
 // headers where they lie under /root/reference/include (oracle/Makefile, target
 // "ref"). Output goes to oracle/_ref/ only. TEST INFRASTRUCTURE: used to pin the
-// oracle restatement and to write tests/golden fixtures; never shipped.
+// oracle restatement, to write tests/golden fixtures and (its -O3 -ffast-math
+// build) to time the CPU baseline. The built library travels with the tree to
+// the GPU host, where only tests and the benchmark's CPU-baseline leg load it;
+// the engine library never links it.
 //
 // It builds a reference WorldState from the same pdpd_world_desc the HIP engine
 // takes, runs one of the reference engines for n steps, and copies x/v/f back.

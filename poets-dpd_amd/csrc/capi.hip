@@ -159,6 +159,8 @@ struct pdpd_ctx {
     DevBuf<float2> cell_par;
     bool spatial = false;                 // some resident world has per-cell strengths
     DevBuf<NextBuffers> next;             // [buffer * 2 + counter copy]: what a chained step's force kernel writes
+    DevBuf<BrickRec> bricks;              // pair-force bricks of the resident worlds
+    uint32_t n_bricks = 0;
 
     // interval snapshot: pos, vel, frc of every resident world (and the flags
     // word) copied on the device behind the enqueued steps; a world's share
@@ -190,6 +192,8 @@ struct pdpd_ctx {
     uint32_t lookback_tiles = 256;
     uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
     uint32_t band_rows = kBandRows;       // force-block order of large worlds (0 = layout order)
+    uint32_t force_kernel = 1;            // 0: per-slot blocks, 1: cell bricks staged in LDS
+    uint32_t brick_shape = 0;             // brick shape index (brick_shape_dims)
 
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
 
@@ -212,6 +216,7 @@ struct pdpd_ctx {
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
         cell_par.release();
         next.release();
+        bricks.release();
         snap.release(); snap_flags.release();
         if (snap_host) (void)hipHostFree(snap_host);
         if (snap_event) (void)hipEventDestroy(snap_event);
@@ -252,6 +257,10 @@ struct pdpd_ctx {
         L.tile_sum = tile_sum.p;
         L.lookback_tiles = lookback_tiles;
         L.spatial = spatial ? 1u : 0u;
+        L.force_kernel = force_kernel;
+        L.brick_shape = brick_shape;
+        L.n_bricks = n_bricks;
+        L.bricks = bricks.p;
         return L;
     }
 };
@@ -292,6 +301,42 @@ void build_blocks(const std::vector<const WorldHost *> &ws, uint32_t band_rows, 
         std::stable_sort(fb.begin() + first, fb.end(),
                          [&](const BeadBlock &a, const BeadBlock &b) { return band_plane(a) < band_plane(b); });
     }
+}
+
+// Pair-force bricks of every resident world: boxes of the shape's cells,
+// clipped at the world's far faces, listed world by world in x, y, z order
+// (the pair-force kernel deals contiguous shares of the list to the XCDs, so
+// an XCD works through neighbouring bricks whose halos overlap in its L2).
+void build_bricks(const std::vector<const WorldHost *> &ws, uint32_t shape, std::vector<BrickRec> &out)
+{
+    uint32_t bx, by, bz;
+    brick_shape_dims(shape, &bx, &by, &bz);
+    out.clear();
+    for (size_t i = 0; i < ws.size(); i++) {
+        const WorldHost &w = *ws[i];
+        if (w.n == 0) continue;
+        for (int x = 0; x < w.L[0]; x += (int)bx)
+            for (int y = 0; y < w.L[1]; y += (int)by)
+                for (int z = 0; z < w.L[2]; z += (int)bz) {
+                    BrickRec r{};
+                    r.world = (int32_t)i;
+                    r.origin = (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+                    const uint32_t dx = std::min<uint32_t>(bx, (uint32_t)(w.L[0] - x));
+                    const uint32_t dy = std::min<uint32_t>(by, (uint32_t)(w.L[1] - y));
+                    const uint32_t dz = std::min<uint32_t>(bz, (uint32_t)(w.L[2] - z));
+                    r.dims = dx | (dy << 8) | (dz << 16);
+                    out.push_back(r);
+                }
+    }
+}
+
+int upload_bricks(pdpd_ctx *ctx, const std::vector<const WorldHost *> &ws)
+{
+    std::vector<BrickRec> br;
+    build_bricks(ws, ctx->brick_shape, br);
+    ctx->n_bricks = (uint32_t)br.size();
+    HIP_TRY(dev_upload(ctx->bricks, br));
+    return PDPD_OK;
 }
 
 // Rebuild the device layout: resident worlds are moved to their new offsets,
@@ -428,6 +473,7 @@ int relayout(pdpd_ctx *ctx)
     ctx->n_fblocks = (uint32_t)fb.size();
     HIP_TRY(dev_upload(ctx->bblocks, bb));
     HIP_TRY(dev_upload(ctx->fblocks, fb));
+    if (int rc = upload_bricks(ctx, ws)) return rc;
 
     // bonded terms with global bead indices; per bead (global index) its terms
     // in the order the per-polymer loop applies them: its bonds, then its
@@ -992,8 +1038,8 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
     HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
     uint32_t flags;
     memcpy(&flags, ctx->snap_host + 3 * n, sizeof(flags));
-    if (flags & PDPD_FLAG_BAD_POSITION)
-        return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
+    if (flags & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
+    if (flags) return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
     const float4 *P = ctx->snap_host, *V = ctx->snap_host + n, *F = ctx->snap_host + 2 * n;
     const double hdt = (double)(float)sw.dt * 0.5;
     for (uint32_t s = 0; s < n; s++) {
@@ -1123,6 +1169,30 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         }
         return PDPD_OK;
     }
+    if (k == "force_kernel") {
+        uint32_t n;
+        if (parse_u32(v, 0, 1, &n)) return fail(PDPD_ERR_INVALID, "force_kernel must be 0 or 1");
+        ctx->force_kernel = n;
+        ctx->graph_version = ~0ull;
+        return PDPD_OK;
+    }
+    if (k == "force_brick") {
+        uint32_t n;
+        if (parse_u32(v, 0, brick_shape_count() - 1, &n))
+            return fail(PDPD_ERR_INVALID, "force_brick must be an integer in [0, " + std::to_string(brick_shape_count() - 1) + "]");
+        if (n != ctx->brick_shape) {
+            ctx->brick_shape = n;
+            ctx->graph_version = ~0ull;
+            if (!ctx->dirty) {     // re-list the bricks of the current layout
+                HIP_TRY(hipSetDevice(ctx->device));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                std::vector<const WorldHost *> ws;
+                for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
+                if (int rc = upload_bricks(ctx, ws)) return rc;
+            }
+        }
+        return PDPD_OK;
+    }
     if (k == "force_variant") {
         uint32_t n;
         if (parse_u32(v, 0, kForceVariants - 1, &n))
@@ -1144,6 +1214,8 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     else if (k == "scan_lookback_tiles") v = std::to_string(ctx->lookback_tiles);
     else if (k == "fused_integrate") v = std::to_string(ctx->fuse_mode);
     else if (k == "force_band_rows") v = std::to_string(ctx->band_rows);
+    else if (k == "force_kernel") v = std::to_string(ctx->force_kernel);
+    else if (k == "force_brick") v = std::to_string(ctx->brick_shape);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -1167,6 +1239,17 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
     s += "force_band_rows=" + std::to_string(ctx->band_rows) +
          "\tpair-force work order of worlds with at least " + std::to_string(kBandMinCells) +
          " cells per y-z plane: bands of this many y rows, plane by plane (0 = layout order)\n";
+    s += "force_kernel=" + std::to_string(ctx->force_kernel) +
+         "\tpair-force kernel: 0 = per-slot blocks of 64 beads, candidates gathered from HBM; 1 = cell bricks"
+         " staged in LDS with their halo (worlds with per-cell strengths always use 0)\n";
+    std::string shapes;
+    for (uint32_t i = 0; i < brick_shape_count(); i++) {
+        uint32_t a, b, c;
+        brick_shape_dims(i, &a, &b, &c);
+        shapes += (i ? ", " : "") + std::to_string(i) + " = " + std::to_string(a) + "x" + std::to_string(b) + "x" +
+                  std::to_string(c);
+    }
+    s += "force_brick=" + std::to_string(ctx->brick_shape) + "\tbrick shape in cells (x, y, z): " + shapes + "\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }

@@ -49,6 +49,17 @@ struct BeadBlock {
     uint32_t pad;
 };
 
+// One pair-force brick: a box of cells of one world, [x0, x0 + bx) x [y0, y0 +
+// by) x [z0, z0 + bz) (clipped at the world's far faces). origin packs x0 | y0
+// << 10 | z0 << 20 (box sides are at most 1024 cells); dims packs bx | by << 8 |
+// bz << 16.
+struct BrickRec {
+    int32_t world;
+    uint32_t origin;
+    uint32_t dims;
+    uint32_t pad;
+};
+
 // Bonded terms: one record per Hookean bond or angle, global bead indices.
 //   kind 0 (Hookean): a = head, b = tail, p0 = r0
 //   kind 1 (angle):   a = head, b = middle, c = tail, p0 = cos(theta0), p1 = sin(theta0)

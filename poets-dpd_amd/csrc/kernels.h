@@ -26,6 +26,10 @@ struct StepLaunch {
     uint32_t *tile_sum;      // scan_tiles(n_cells) entries: tile sums of the three-phase scan
     uint32_t lookback_tiles; // single-pass look-back scan up to this many tiles, three-phase above
     uint32_t spatial;        // some world has per-cell strengths (the pair-force kernel with the cell lookup)
+    uint32_t force_kernel;   // 0: per-slot blocks (k_force), 1: cell bricks staged in LDS (k_force_brick)
+    uint32_t brick_shape;    // brick shape index (brick_shape_dims)
+    uint32_t n_bricks;
+    const BrickRec *bricks;
 };
 
 constexpr uint32_t kForceVariants = 4;
@@ -41,6 +45,11 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev);
 // The pair-force launch of the step L describes (reads and writes buffer
 // L.cur ^ 1; the bonded terms were evaluated during the step's scatter).
 hipError_t launch_force(const StepLaunch &L, hipStream_t st);
+// The pair-force launch over cell bricks (force_brick.hip); same inputs, outputs
+// and bits as launch_force's per-slot kernel.
+hipError_t launch_force_brick(const StepLaunch &L, hipStream_t st);
+uint32_t brick_shape_count();
+void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
 // Pair records of world `world` (n_beads beads) from buffer buf, step counter t; appended at *count.

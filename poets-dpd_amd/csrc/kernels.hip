@@ -32,57 +32,11 @@
 #include "device_types.h"
 #include "kernels.h"
 #include "pdpd_hip.h"
+#include "step_device.h"
 
 #include <algorithm>
 
 namespace pdpd {
-
-constexpr float kMinR2 = 1e-9f * 1e-9f;           // MIN_DISTANCE_CUTOFF_SQR
-constexpr float kTwoM32 = 0.00000000023283064365386962890625f;  // 2^-32
-constexpr uint32_t kInactive = 0x80000000u;
-constexpr uint32_t kSlotMask = kMaxLayoutBeads - 1;
-constexpr unsigned long long kTileAgg = 1ull << 32, kTileIncl = 2ull << 32;
-constexpr uint32_t kScanSpinLimit = 1u << 24;
-
-// ---------------------------------------------------------------- RNG ----
-
-__device__ __forceinline__ uint64_t mix64_m2(uint64_t x)
-{
-    const uint64_t C = 0xd392d2a7ull;
-    x ^= x >> 32;
-    x *= C;
-    x ^= x >> 32;
-    x *= C;
-    x ^= x >> 32;
-    return x;
-}
-
-__device__ __forceinline__ uint64_t step_hash(uint32_t t, uint32_t seed_lo, uint32_t seed_hi)
-{
-    const uint64_t seed = ((uint64_t)seed_hi << 32) | seed_lo;
-    return mix64_m2(seed + mix64_m2((uint64_t)t));
-}
-
-// hash_rng_sym(t_hash, a, b): low word of mix64_m2((min | max << 32) ^ t_hash),
-// carried in 32-bit halves: each 64x32 product is mul_lo + mul_hi + mul_lo.
-__device__ __forceinline__ uint32_t rng_sym(uint32_t th_lo, uint32_t th_hi, uint32_t a, uint32_t b)
-{
-    const uint32_t C = 0xd392d2a7u;
-    uint32_t lo = min(a, b) ^ th_lo;
-    uint32_t hi = max(a, b) ^ th_hi;
-    lo ^= hi;
-    uint32_t l1 = lo * C;
-    uint32_t h1 = __umulhi(lo, C) + hi * C;
-    l1 ^= h1;
-    uint32_t l2 = l1 * C;
-    uint32_t h2 = __umulhi(l1, C) + h1 * C;
-    return l2 ^ h2;
-}
-
-__device__ __forceinline__ float rng_uniform(uint32_t r)
-{
-    return (float)(int32_t)r * kTwoM32;
-}
 
 __global__ void k_debug_rng(uint32_t n, const uint64_t *__restrict__ th, const uint32_t *__restrict__ a,
                             const uint32_t *__restrict__ b, uint32_t *__restrict__ u32, float *__restrict__ f32)
@@ -93,117 +47,6 @@ __global__ void k_debug_rng(uint32_t n, const uint64_t *__restrict__ th, const u
     uint32_t r = rng_sym((uint32_t)t, (uint32_t)(t >> 32), a[i], b[i]);
     u32[i] = r;
     f32[i] = rng_uniform(r);
-}
-
-// ------------------------------------------------------------ helpers ----
-
-__device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
-
-// Exclusive scan of one value per thread across a block of NW wave64s.
-// Returns the exclusive prefix; *total receives the block sum.
-template <int NW>
-__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total)
-{
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wsum[wid] = x;
-    __syncthreads();
-    uint32_t base = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < NW; k++) {
-        uint32_t s = s_wsum[k];
-        if (k < wid) base += s;
-        tot += s;
-    }
-    __syncthreads();
-    *total = tot;
-    return base + x - v;
-}
-
-__device__ __forceinline__ int cell_coord(float x, int L)
-{
-    int c = (int)floorf(x);
-    c = c < 0 ? 0 : c;
-    return c >= L ? L - 1 : c;
-}
-
-// ---------------------------------------------------------- integrate ----
-
-// One bead's finish-kick, drift, half-kick and wrap from its force f:
-// v(t) = w + f dt/2, v(t+dt/2) = v(t) + f dt/2, x(t+dt) = x + dt v(t+dt/2),
-// then w = v(t+dt/2). A bead of a stationary type keeps x; the pair force sees
-// v(t) (the imported v on the first step, 0 after) --
-// naive_dpd_engine_half_merge.hpp:327-331.
-__device__ __forceinline__ void drift_bead(const WorldDev *__restrict__ W, float4 &p, float4 &v, const float4 f)
-{
-    const float dt = W->dt, hdt = W->hdt;
-    const float fv[3] = {f.x, f.y, f.z};
-    float x[3] = {p.x, p.y, p.z};
-    float wv[3] = {v.x, v.y, v.z};
-    if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) wv[d] = fmaf(fv[d], hdt, wv[d]);
-    } else {
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const float Lf = W->Lf[d];
-            float vv = fmaf(fv[d], hdt, wv[d]);     // v(t)       = w + f dt/2
-            float vh = fmaf(fv[d], hdt, vv);        // v(t+dt/2)  = v + f dt/2
-            float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
-            xn += (xn < 0.0f) ? Lf : 0.0f;
-            xn -= (xn >= Lf) ? Lf : 0.0f;
-            x[d] = xn;
-            wv[d] = vh;
-        }
-    }
-    p = make_float4(x[0], x[1], x[2], p.w);
-    v = make_float4(wv[0], wv[1], wv[2], v.w);
-}
-
-// Runs of equal keys in consecutive lanes of one wave (called converged; the
-// valid lanes are a prefix and hold consecutive slots in the last step's cell
-// order, so beads of one cell come in runs): for each valid lane, the first
-// lane of its run (start) and the run's length from this lane on.
-__device__ __forceinline__ void key_runs(uint32_t key, bool valid, int &start, int &next, bool &head)
-{
-    const int lane = threadIdx.x & 63;
-    const uint32_t prev = __shfl_up(key, 1, 64);
-    head = valid && (lane == 0 || prev != key);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long lanes = __ballot(valid);
-    const unsigned long long upto = (2ull << lane) - 1ull;          // lanes 0..lane
-    start = 63 - __clzll((long long)(heads & upto));                 // first lane of this run
-    const unsigned long long above = heads & ~upto;
-    next = above ? __ffsll((long long)above) - 1 : (int)__popcll(lanes);
-}
-
-// The cell key of the bead in slot s at position p (with kInactive for a
-// bead that does not move this step), and its count in cell_count: one
-// atomic per run of equal keys, whose result is not waited for. Raises
-// PDPD_FLAG_BAD_POSITION for a position outside the box. Called by all lanes
-// of a wave, converged.
-__device__ __forceinline__ void count_bead(uint32_t *__restrict__ flags, uint32_t *__restrict__ cell_count,
-                                           uint32_t *__restrict__ key_out, const WorldDev *__restrict__ W, uint32_t s,
-                                           const float4 p, bool active, bool valid)
-{
-    const float x[3] = {p.x, p.y, p.z};
-    bool bad = false;
-#pragma unroll
-    for (int d = 0; d < 3; d++) bad |= !(x[d] >= 0.0f && x[d] < W->Lf[d]);
-    if (valid && bad) atomicOr(flags, PDPD_FLAG_BAD_POSITION);
-    const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
-    const uint32_t key = valid ? W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2) : 0xffffffffu;
-    int start, next;
-    bool head;
-    key_runs(key, valid, start, next, head);
-    if (head) atomicAdd(&cell_count[key], (uint32_t)(next - (int)(threadIdx.x & 63)));
-    if (valid) key_out[s] = key | (active ? 0u : kInactive);
 }
 
 __device__ __forceinline__ void reset_scan_state(const DeviceState &S, uint32_t n_tiles, int tid, int nthreads)
@@ -547,19 +390,6 @@ __device__ __forceinline__ float shift_of(uint32_t code, float L)
 
 constexpr int kQueue = 256;       // accepted pairs buffered per pair wave before they are evaluated
 constexpr int kLaneList = 16;     // accepted pairs buffered per home bead (queue positions, 8 bits)
-constexpr uint32_t kXcds = 8;     // MI355X: 8 accelerator complex dies, each with its own L2
-
-// Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD i mod 8).
-// Map logical index i to a work item so that each XCD takes one contiguous
-// share of the (cell-ordered) work list and its L2 holds one region of the
-// world instead of all of it. A bijection on [0, n); with a grid-stride loop
-// whose stride is a multiple of 8, a workgroup stays on its XCD's share.
-__device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t n)
-{
-    const uint32_t q = n / kXcds, r = n % kXcds;
-    const uint32_t x = i % kXcds, k = i / kXcds;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
-}
 
 // Slots are below 2^26, so the byte offset of a float4 fits 32 bits: the loads
 // take a scalar base and a 32-bit vector offset.
@@ -574,83 +404,18 @@ __device__ __forceinline__ float3 load_xyz(const float4 *__restrict__ p, uint32_
     return make_float3(q[0], q[1], q[2]);
 }
 
-// The bonded terms of the bead in slot s (evaluated during k_scatter), summed in FP64
-// in the per-polymer order into s_bond / s_nterms (lane = the bead's lane in
-// its workgroup).
+// The bonded terms of the bead in slot s, summed in FP64 into s_bond /
+// s_nterms (lane = the bead's lane in its workgroup).
 __device__ __forceinline__ void bonded_terms(const DeviceState &S, const float4 *__restrict__ vel, uint32_t s,
                                              bool moving, int lane, double (*s_bond)[kForceBeads],
                                              uint32_t *s_nterms)
 {
     double bx = 0.0, by = 0.0, bz = 0.0;
-    uint32_t nt = 0;
-    if (moving) {
-        const uint32_t b = fbits(vel[s].w);
-        const uint32_t t0 = S.term_start[b], t1 = S.term_start[b + 1];
-        nt = t1 - t0;
-        for (uint32_t t = t0; t < t1; t++) {
-            const uint32_t e = S.bead_terms[t];
-            const uint32_t role = e >> kRoleShift;
-            const double *__restrict__ F = S.term_frc + (size_t)(e & ((1u << kRoleShift) - 1u)) * kTermFrcStride;
-            double cx, cy, cz;
-            if (role == kRoleBondHead) { cx = -F[0]; cy = -F[1]; cz = -F[2]; }
-            else if (role == kRoleBondTail || role == kRoleAngleHead) { cx = F[0]; cy = F[1]; cz = F[2]; }
-            else if (role == kRoleAngleTail) { cx = F[3]; cy = F[4]; cz = F[5]; }
-            else { cx = -(F[0] + F[3]); cy = -(F[1] + F[4]); cz = -(F[2] + F[5]); }
-            bx += cx; by += cy; bz += cz;
-        }
-    }
+    const uint32_t nt = moving ? bonded_sum(S, fbits(vel[s].w), bx, by, bz) : 0u;
     s_bond[0][lane] = bx;
     s_bond[1][lane] = by;
     s_bond[2][lane] = bz;
     s_nterms[lane] = nt;
-}
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
-    return v;
-}
-
-// Orders this wave's LDS writes before its later LDS reads by other lanes.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// The next step's integration, done by the pair-force kernel for every step of
-// a chain but the last. From the position, half-step velocity and the force
-// just computed, each bead's drifted state goes into buffer nxt ^ 1 (the
-// step's source buffer, free after the sort) at the same slot, with its cell
-// key and count: what k_integrate would compute at the start of the
-// next step, so the next step starts at its scan. Chains are only formed over
-// steps in which every resident world steps (no step budget runs out inside
-// a chain but at its end), so every bead is active in both steps.
-// The kernel reads the next step's buffers through an entry of a table in
-// device memory (null: not chained), after its pair loop, so that their
-// addresses take no scalar registers during the loop.
-
-// One bead (all lanes of the wave call it, the valid ones a prefix).
-__device__ __forceinline__ void advance_next(const NextBuffers &B, const WorldDev *__restrict__ W, uint32_t s,
-                                             bool valid, float4 p, float4 v, const float4 f)
-{
-    if (valid) {
-        // k_stationary zeroes the velocity of a stationary bead after this kernel
-        if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) v = make_float4(0.0f, 0.0f, 0.0f, v.w);
-        drift_bead(W, p, v, f);
-        B.pos[s] = p;
-        B.vel[s] = v;
-    }
-    count_bead(B.flags, B.cell_count, B.key, W, s, p, true, valid);
-}
-
-// The table entry, read after a compiler barrier so the loads stay here.
-__device__ __forceinline__ NextBuffers next_buffers(const NextBuffers *b)
-{
-    __asm__ volatile("" ::: "memory");
-    return *b;
 }
 
 // Pair forces. A workgroup takes 64 consecutive slots of one world (one home
@@ -1119,6 +884,9 @@ hipError_t launch_step(const StepLaunch &L, hipStream_t st, KernelEvents *ev)
 
 hipError_t launch_force(const StepLaunch &L, hipStream_t st)
 {
+    // cell bricks staged in LDS, unless a world has per-cell strengths (the
+    // per-slot kernel carries that lookup)
+    if (L.force_kernel == 1 && !L.spatial && L.bricks) return launch_force_brick(L, st);
     DeviceState Sp = L.S;
     Sp.ctr = L.S.ctr + L.cpar * L.n_worlds;      // counters read by that step
     const int nxt = L.cur ^ 1;

@@ -179,6 +179,7 @@ int pdpd_server_submit(pdpd_server *s, const pdpd_world_desc *w, double arrival_
     if (!s || !w) return sfail(PDPD_ERR_INVALID, "null argument");
     const char *why = pdpd_can_support(w);
     if (why[0]) return sfail(PDPD_ERR_UNSUPPORTED, why);
+    if (w->n_beads == 0) return sfail(PDPD_ERR_INVALID, "request has no beads");
     std::unique_ptr<Request> r(new Request());
     r->id = s->next_id++;
     r->desc.reset(new OwnedDesc(*w));

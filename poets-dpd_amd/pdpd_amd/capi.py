@@ -448,6 +448,10 @@ class Ensemble:
         for c in self.parts:
             c.set_config(key, value)
 
+    def get_config(self, key):
+        """The setting of the first context (set_config sets every context alike)."""
+        return self.parts[0].get_config(key)
+
     def autotune(self, steps_per_trial=20):
         """The launch shape of a single-context ensemble. With several contexts
         the default shapes are kept: the autotuner times a context alone, and

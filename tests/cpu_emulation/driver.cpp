@@ -140,12 +140,18 @@ int main(int argc, char **argv)
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
+        {"dense_brick3", {5, 4, 6}, 0.0, false, 0.0, 2, "force_brick=3"},
+        {"poly_brick1", {6, 7, 8}, 0.2, false, 0.0, 2, "force_brick=1"},
+        {"wall_brick2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_brick=2"},
+        {"poly_brick5", {9, 8, 10}, 0.1, false, 0.0, 2, "force_brick=5"},
+        {"poly_brick_grid", {9, 8, 10}, 0.1, false, 0.0, 3, "force_grid=3"},
+        {"poly_per_slot", {6, 7, 8}, 0.1, false, 0.0, 2, "force_kernel=0"},
         {"spatial", {5, 4, 6}, 0.2, true, 0.5, 2, nullptr},      // strengths per cell, batched with a constant world
     };
     for (auto &c : cases) {
         fprintf(stderr, "case %s\n", c.name);
         W w;
-        make(w, c.L[0], c.L[1], c.L[2], c.bonded, c.wall, c.th, 7, strcmp(c.name, "dense") ? 3 : 24);
+        make(w, c.L[0], c.L[1], c.L[2], c.bonded, c.wall, c.th, 7, strncmp(c.name, "dense", 5) ? 3 : 24);
         if (!strcmp(c.name, "spatial")) {
             const int nt = (int)w.d.n_bead_types, nc = c.L[0] * c.L[1] * c.L[2];
             for (int cell = 0; cell < nc; cell++)

@@ -13,6 +13,7 @@ and WorldState. It prints TAP. Two suites:
 """
 import os
 import subprocess
+import threading
 
 import pytest
 
@@ -26,23 +27,36 @@ FAMILIES = ["TestIsolatedMovingBead", "TestMultipleMovingBeads", "TestCollidingB
 MIN_TESTS = {"scenarios": 200, "differential": 200}      # the registered suite holds 225 scenarios
 
 
-def _run(mode, family):
+def _run(mode, family, limit_s=100.0):
     """Run one scenario family (name prefix up to '['), streaming each TAP line as
-    it arrives so a long family shows progress."""
+    it arrives so a long family shows progress. stderr is merged into stdout; a
+    reader thread collects the lines and the child is killed at the deadline,
+    so a hung or chatty child fails the test instead of blocking it."""
     if not os.path.exists(BIN):
         pytest.skip("oracle/_ref/ref_engine_tests not built (needs the reference tree at build time)")
     print(f"reference harness: {mode} {family}", flush=True)
-    p = subprocess.Popen([BIN, mode, family + "["], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
-    lines = []
-    for line in p.stdout:
-        if line.startswith("ok ") or line.startswith("not ok "):
-            lines.append(line.rstrip())
-            print("  " + line.rstrip()[:160], flush=True)
-    err = p.stderr.read()
-    rc = p.wait(timeout=60)
+    p = subprocess.Popen([BIN, mode, family + "["], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    lines, other = [], []
+
+    def reader():
+        for line in p.stdout:
+            if line.startswith("ok ") or line.startswith("not ok "):
+                lines.append(line.rstrip())
+                print("  " + line.rstrip()[:160], flush=True)
+            else:
+                other.append(line)
+    t = threading.Thread(target=reader, daemon=True)
+    t.start()
+    t.join(limit_s)
+    if t.is_alive():
+        p.kill()
+        p.wait()
+        t.join(5)
+        pytest.fail(f"{mode} {family}: no end after {limit_s} s ({len(lines)} tests reported)")
+    rc = p.wait(timeout=10)
     failed = [l for l in lines if l.startswith("not ok")]
     print(f"  {len(lines)} tests, {len(failed)} failed", flush=True)
-    return rc, lines, failed, err
+    return rc, lines, failed, "".join(other)
 
 
 _counts = {}

@@ -10,8 +10,10 @@ live work-items, and a wave intrinsic reached by only part of a wave aborts.
   * the plain build's first step equals the FP32 oracle bit for bit (the same
     source as the gfx950 build, FP32 with -ffp-contract=off), including a dense
     world whose accepted pairs overflow the pair kernel's per-bead lists and
-    per-wave queues many times, a world whose strengths vary per cell, and that
-    world batched with a constant-strength one.
+    per-wave queues many times (and whose bricks exceed the LDS capacity of
+    the brick kernel), every brick shape, the per-slot kernel, a world whose
+    strengths vary per cell, and that world batched with a constant-strength
+    one.
 This checks the kernel source on the CPU; GPU parity is tests/test_gpu_*.py."""
 import os
 import subprocess
@@ -75,6 +77,30 @@ for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2, 1)):
             np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
     print(name, "bitwise" if same else "DIFFERENT", flush=True)
     ctx.close()
+# every brick shape of the LDS brick kernel (the dense world takes the path for
+# bricks beyond the LDS capacity), and the per-slot kernel
+for shape in range(6):
+    same = True
+    for name in ("poly_6x7x8", "wall_bent6", "dense3", "lattice4"):
+        ctx = capi.Context(0)
+        ctx.set_config("force_brick", shape)
+        slot = ctx.add(worlds[name])
+        ctx.run(2)
+        x, ww, f, t = ctx.download_f32(slot)
+        ref = O.F32State(worlds[name]).step(2)
+        same &= (np.array_equal(x.view(np.uint32), ref.x.view(np.uint32)) and
+                 np.array_equal(ww.view(np.uint32), ref.w.view(np.uint32)) and
+                 np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
+        ctx.close()
+    print(f"brick{shape}", "bitwise" if same else "DIFFERENT", flush=True)
+ctx = capi.Context(0)
+ctx.set_config("force_kernel", 0)
+slot = ctx.add(worlds["poly_6x7x8"])
+ctx.run(2)
+x, ww, f, t = ctx.download_f32(slot)
+ref = O.F32State(worlds["poly_6x7x8"]).step(2)
+print("per_slot", "bitwise" if np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)) else "DIFFERENT", flush=True)
+ctx.close()
 # one context with a spatial and a constant world: each world's own strengths
 ctx = capi.Context(0)
 slots = [ctx.add(worlds["spatial"]), ctx.add(worlds["water4"])]
@@ -114,7 +140,7 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 9, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 16, r.stdout
 
 
 SERVE_PROBE = r"""
