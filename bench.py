@@ -290,6 +290,8 @@ def main():
                     help="run one workload only (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
+    ap.add_argument("--config", action="append", default=[], metavar="KEY=VALUE",
+                    help="engine setting for every context (pdpd_set_config), applied after autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB, 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -313,6 +315,8 @@ def main():
         ctx.add(world)
         if args.graph_steps is not None:
             ctx.set_config("graph_steps", str(args.graph_steps))
+        for kv in args.config:
+            ctx.set_config(*kv.split("=", 1))
         warm(ctx, args.warmup)
         el = timed(ctx, args.steps)
         log(f"bench: {args.only}: {world.n_beads * args.steps / el:.4e} bead-steps/s")
@@ -327,6 +331,9 @@ def main():
     if args.graph_steps is not None:
         ens.set_config("graph_steps", str(args.graph_steps))
     tuning = "default launch shape" if args.no_autotune else ens.autotune(10)
+    for kv in args.config:
+        ens.set_config(*kv.split("=", 1))
+        tuning += f" {kv}"
     log(f"bench: tuning: {tuning}")
     if args.only == "ensemble":
         warm(ens, args.warmup)

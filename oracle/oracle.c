@@ -450,9 +450,9 @@ typedef struct f32_pair_job {
     int begin, end;                /* beads [begin, end) of the cell order */
 } f32_pair_job;
 
-/* Pair force of bead i: one partial sum per (x-plane, y-row) of the stencil,
- * summed over the row's cells in z order (z-1, z, z+1) and slot order; rows
- * combined (r0 + r1) + r2 per plane, planes (p0 + p1) + p2.
+/* Pair force of bead i: one running sum per x-plane of the stencil over the
+ * plane's rows (dy = -1, 0, +1) in order, each row's cells in z order (z-1, z,
+ * z+1) and each cell in slot order; planes combined (p0 + p1) + p2.
  * dpd_maths_core_half_step_raw.hpp:129-216 with the terms grouped as
  * a wr + sgw (scale u - sgw rdotv). */
 static void f32_pair_force_bead(const f32_pair_job *J, int k0)
@@ -475,9 +475,8 @@ static void f32_pair_force_bead(const f32_pair_job *J, int k0)
     const int htype = (int)(hh >> 28);
     float px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, pz[3] = {0, 0, 0}, fa = 0.0f;
     for (int ddx = -1; ddx <= 1; ddx++) {
-    float rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0};
-    for (int ddy = -1; ddy <= 1; ddy++) {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int ddy = -1; ddy <= 1; ddy++) {
     for (int ddz = -1; ddz <= 1; ddz++) {
         int oc3[3];
         float sh[3];
@@ -527,13 +526,10 @@ static void f32_pair_force_bead(const f32_pair_job *J, int k0)
             fa += fabsf(F);
         }
     }
-    rx[ddy + 1] = fx;
-    ry[ddy + 1] = fy;
-    rz[ddy + 1] = fz;
     }
-    px[ddx + 1] = (rx[0] + rx[1]) + rx[2];
-    py[ddx + 1] = (ry[0] + ry[1]) + ry[2];
-    pz[ddx + 1] = (rz[0] + rz[1]) + rz[2];
+    px[ddx + 1] = fx;
+    py[ddx + 1] = fy;
+    pz[ddx + 1] = fz;
     }
     f_out[3 * i] = (px[0] + px[1]) + px[2];
     f_out[3 * i + 1] = (py[0] + py[1]) + py[2];

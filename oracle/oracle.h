@@ -23,7 +23,9 @@
  *                   integrate (dpd_maths_core_half_step_raw.hpp:42-72,114-124 with
  *                   the AVX2 half-kick shift of dpd_engine_avx2_half_merge_tbb.hpp:
  *                   524,554), cell key + stable cell order, the full 27-cell pair
- *                   force of dpd_maths_core_half_step_raw.hpp:129-216, Hookean bonds
+ *                   force of dpd_maths_core_half_step_raw.hpp:129-216 (summed per
+ *                   bead as one running sum per stencil x-plane, rows and cells
+ *                   in order, planes (p0 + p1) + p2), Hookean bonds
  *                   (naive_dpd_engine_half_merge.hpp:638-654) and angle bonds
  *                   (dpd_maths_core_half_step.hpp:178-231). Pinned against the FP64
  *                   naive restatement (within FP32 error) and the invariants in

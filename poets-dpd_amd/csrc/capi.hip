@@ -192,7 +192,7 @@ struct pdpd_ctx {
     uint32_t lookback_tiles = 256;
     uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
     uint32_t band_rows = kBandRows;       // force-block order of large worlds (0 = layout order)
-    uint32_t force_kernel = 1;            // 0: per-slot blocks, 1: cell bricks staged in LDS
+    uint32_t force_kernel = 0;            // 0: per-slot blocks, 1: cell bricks staged in LDS (autotuned)
     uint32_t brick_shape = 0;             // brick shape index (brick_shape_dims)
 
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
@@ -1390,8 +1390,9 @@ int pdpd_trace_forces(pdpd_ctx *ctx, int slot, pdpd_pair_record *pairs, uint32_t
     return PDPD_OK;
 }
 
-// Times launch shapes that do not change results (the pair-force grid and the
-// graph length) on a saved copy of the state, keeps the fastest.
+// Times launch shapes that do not change results (the pair-force kernel,
+// its grid, variant or brick shape, and the graph length) on a saved copy of
+// the state, keeps the fastest.
 int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len)
 {
     if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
@@ -1434,13 +1435,27 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     const uint32_t grids[] = {0u, (uint32_t)cus * 4, (uint32_t)cus * 8, (uint32_t)cus * 16};
     const uint32_t graphs[] = {10u, 20u, 50u};
     uint32_t best_grid = ctx->force_grid, best_graph = ctx->graph_steps, best_var = ctx->force_variant;
+    uint32_t best_kernel = ctx->force_kernel, best_shape = ctx->brick_shape;
     const uint32_t fuse0 = ctx->fuse() ? 1u : 0u;
     uint32_t best_fuse = fuse0;
     double best_ms = 1e30;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    std::vector<const WorldHost *> ws;
+    for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
+    auto set_shape = [&](uint32_t kernel, uint32_t shape) -> int {
+        ctx->force_kernel = kernel;
+        if (shape != ctx->brick_shape) {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            ctx->brick_shape = shape;
+            if (int r = upload_bricks(ctx, ws)) return r;
+        }
+        return PDPD_OK;
+    };
+    uint32_t cur_kernel = ctx->force_kernel, cur_shape = ctx->brick_shape;
     auto trial = [&](uint32_t g, uint32_t var, uint32_t gs, uint32_t fz) -> int {
+        if (int r = set_shape(cur_kernel, cur_shape)) return r;
         ctx->force_grid = g;
         ctx->force_variant = var;
         ctx->graph_steps = gs;
@@ -1459,14 +1474,28 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         const double per = ms / steps;
-        if (per < best_ms) { best_ms = per; best_grid = g; best_var = var; best_graph = gs; best_fuse = fz; }
+        if (per < best_ms) {
+            best_ms = per; best_grid = g; best_var = var; best_graph = gs; best_fuse = fz;
+            best_kernel = cur_kernel; best_shape = cur_shape;
+        }
         return PDPD_OK;
     };
-    // pair-force grid x kernel variant at 20 steps per graph, then the graph
-    // length with chained steps off and on
+    // pair-force kernels: the per-slot kernel over grid x variant, the brick
+    // kernel over its shapes (one workgroup per brick), at 20 steps per graph;
+    // then the graph length with chained steps off and on
+    cur_kernel = 0;
     for (uint32_t g : grids)
         for (uint32_t var = 0; var < kForceVariants; var++)
             if ((rc = trial(g, var, 20u, fuse0))) return rc;
+    if (!ctx->spatial) {
+        cur_kernel = 1;
+        for (uint32_t sh = 0; sh < brick_shape_count(); sh++) {
+            cur_shape = sh;
+            if ((rc = trial(0u, ctx->force_variant, 20u, fuse0))) return rc;
+        }
+    }
+    cur_kernel = best_kernel;
+    cur_shape = best_shape;
     const uint32_t g_best = best_grid, v_best = best_var;
     for (uint32_t gs : graphs)
         for (uint32_t fz = 0; fz < 2; fz++)
@@ -1477,6 +1506,7 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     sp.release(); sv.release(); sf.release(); ss.release();
     ctx->steps_on_layout = 0;      // the cell lists are the last trial's, not the restored state's
+    if ((rc = set_shape(best_kernel, best_shape))) return rc;
     ctx->force_grid = best_grid;
     ctx->force_variant = best_var;
     ctx->graph_steps = best_graph;
@@ -1484,8 +1514,10 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     ctx->graph_version = ~0ull;
     if ((rc = check_flags(ctx))) return rc;
     if (choice && choice_len)
-        snprintf(choice, choice_len, "force_grid=%u force_variant=%u graph_steps=%u fused_integrate=%u ms_per_step=%.5f",
-                 best_grid, best_var, best_graph, best_fuse, best_ms);
+        snprintf(choice, choice_len,
+                 "force_kernel=%u force_brick=%u force_grid=%u force_variant=%u graph_steps=%u fused_integrate=%u "
+                 "ms_per_step=%.5f",
+                 best_kernel, best_shape, best_grid, best_var, best_graph, best_fuse, best_ms);
     return PDPD_OK;
 }
 

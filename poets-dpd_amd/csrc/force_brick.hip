@@ -16,13 +16,13 @@
 // length. Accepted candidates go to a per-wave queue; when it fills, all lanes
 // evaluate queued pairs side by side (dpd_maths_core_half_step_raw.hpp:129-216,
 // RNG of hash.hpp:61-88 + dpd_maths_core.hpp:55-79) and each lane then adds its
-// own pairs to its row sums in acceptance order.
+// own pairs to its plane sum in acceptance order.
 //
-// Summation order per bead (the oracle's): a sum per (plane, row) over the
-// row's candidates in z-cell then slot order; rows (r0 + r1) + r2 per plane;
-// planes (p0 + p1) + p2; then the FP64 bonded sum, rounded once. Which lane or
-// wave evaluates a pair and when the queue flushes change nothing in it, so
-// forces are bit-identical to the per-slot kernel's and the oracle's.
+// Summation order per bead (the oracle's): one running sum per stencil
+// plane over its rows' candidates in row, z-cell and slot order; planes
+// (p0 + p1) + p2; then the FP64 bonded sum, rounded once. Which lane or wave
+// evaluates a pair and when the queue flushes change nothing in it, so forces
+// are bit-identical to the per-slot kernel's and the oracle's.
 //
 // A brick whose halo or home beads exceed the LDS capacity (dense regions)
 // takes a slower path in the same workgroup: its home cells one at a time, up
@@ -39,7 +39,7 @@ namespace {
 
 constexpr int kBins = 64;          // counting-sort bins of task lengths (longer tasks share the last)
 constexpr int kList = 16;          // accepted pairs listed per lane between flushes (queue positions, 8 bits)
-constexpr int kUnroll = 2;         // candidates per lane per iteration
+constexpr int kUnroll = 4;         // candidates per lane per iteration
 
 // Halo cell (i, j, k), k fastest, of a halo of dims hy x hz per plane.
 __device__ __forceinline__ int halo_cell(int i, int j, int k, int hy, int hz) { return (i * hy + j) * hz + k; }
@@ -61,12 +61,12 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
 {
     constexpr int NT = NW * 64;
     constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
-    constexpr int HC = HX * HY * HZ;                 // halo cells at most
-    constexpr int NHC = BX * BY * BZ;                // home cells at most
-    constexpr int CPT = (HC + NT - 1) / NT;          // halo cells per thread in the scan
-    constexpr int HPT = (NHC + NT - 1) / NT;         // home cells per thread in the scan
+    constexpr int HC = HX * HY * HZ;                 // halo cells (strides fixed by the shape; clipped cells are empty)
+    constexpr int NHC = BX * BY * BZ;                // home cells
+    constexpr int CPT = (HC + NT - 1) / NT;          // halo cells per thread
+    constexpr int HPT = (NHC + NT - 1) / NT;         // home cells per thread
     constexpr int kDense = NT;                       // dense path: home beads per pass (one per thread)
-    static_assert(QCAP >= 64 * kUnroll && QCAP <= 256, "queue positions are 8-bit; an emptied queue holds a batch");
+    static_assert(QCAP >= 64 && QCAP <= 256, "queue positions are 8-bit; an emptied queue holds one candidate per lane");
     static_assert(kList >= kUnroll, "a lane lists up to kUnroll pairs per batch");
     static_assert(HCAP < 65536 && 3 * HOMECAP < 65536, "16-bit LDS indices");
     static_assert(HCAP >= kDense + 64, "the dense path stages home beads and a chunk of candidates");
@@ -77,15 +77,15 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
     __shared__ uint32_t s_gstart[HC];                // global slot of each halo cell's first bead
     __shared__ uint16_t s_hidx[HOMECAP];             // home bead -> LDS index
     __shared__ uint16_t s_hcell[HOMECAP];            // home bead -> halo cell
-    __shared__ uint16_t s_task[3 * HOMECAP];         // tasks (home bead * 3 + plane) by length
+    __shared__ uint16_t s_task[3 * HOMECAP];         // tasks sorted by length, longest first
     __shared__ uint32_t s_bin[kBins];
     __shared__ float s_part[3][3][HOMECAP];          // plane sums [plane][axis][home bead]
     __shared__ uint32_t s_q[NW][QCAP];               // queued pairs: candidate | home << 16 (LDS indices)
     __shared__ float4 s_qres[NW][QCAP];              // their (unit vector, force magnitude)
-    __shared__ uint8_t s_list[kList][NT];            // per lane its queued pairs in acceptance order
+    __shared__ uint8_t s_list[kList][NT];            // per lane its queued pairs (queue positions) in order
     __shared__ float s_a[kMaxTypes * kMaxTypes], s_sg[kMaxTypes * kMaxTypes];
     __shared__ uint32_t s_wsum[NW];
-    __shared__ uint32_t s_misc[4];                   // halo beads, home beads, tasks, next chunk
+    __shared__ uint32_t s_misc[4];                   // -, -, tasks, next chunk
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -110,8 +110,6 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
         const WorldDev *__restrict__ W = &S.worlds[br.world];
         const int x0 = (int)(br.origin & 1023u), y0 = (int)((br.origin >> 10) & 1023u), z0 = (int)(br.origin >> 20);
         const int bx = (int)(br.dims & 255u), by = (int)((br.dims >> 8) & 255u), bz = (int)((br.dims >> 16) & 255u);
-        const int hy = by + 2, hz = bz + 2;
-        const int hc = (bx + 2) * hy * hz;
         const int L0 = W->L[0], L1 = W->L[1], L2 = W->L[2];
         const float Lx = W->Lf[0], Ly = W->Lf[1], Lz = W->Lf[2];
         {   // the world's n_types^2 strengths (rows of kMaxTypes)
@@ -127,15 +125,15 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
         const uint32_t th_lo = (uint32_t)th, th_hi = (uint32_t)(th >> 32);
         const float scale = W->rng_scale;
 
-        // ---- halo cell table: global starts, counts, LDS starts (block scan)
+        // ---- halo cells (thread c: cells c * CPT ...): global starts, counts, LDS starts
         uint32_t cnt[CPT];
         uint32_t mysum = 0;
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             const int c = tid * CPT + q;
+            const int k = c % HZ, j = (c / HZ) % HY, i = c / (HZ * HY);
             cnt[q] = 0;
-            if (c < hc) {
-                const int k = c % hz, j = (c / hz) % hy, i = c / (hz * hy);
+            if (c < HC && i < bx + 2 && j < by + 2 && k < bz + 2) {
                 const int gx = wrap(x0 - 1 + i, L0), gy = wrap(y0 - 1 + j, L1), gz = wrap(z0 - 1 + k, L2);
                 const uint32_t id = W->cell_off + (uint32_t)((gx * L1 + gy) * L2 + gz);
                 const uint32_t a = cs[id], b = cs[id + 1];
@@ -145,32 +143,34 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             mysum += cnt[q];
         }
         uint32_t total;
-        uint32_t ex = block_scan<NW>(mysum, s_wsum, &total);
+        const uint32_t ex0 = block_scan<NW>(mysum, s_wsum, &total);
+        {
+            uint32_t ex = ex0;
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int c = tid * CPT + q;
-            if (c < hc) s_cstart[c] = ex;
-            ex += cnt[q];
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid * CPT + q;
+                if (c < HC) s_cstart[c] = ex;
+                ex += cnt[q];
+            }
         }
-        if (tid == 0) s_cstart[hc] = total;
+        if (tid == 0) s_cstart[HC] = total;
+        __syncthreads();
         // home beads: home cells (1..bx, 1..by, 1..bz) in halo order
-        const int nhc = bx * by * bz;
         uint32_t hcnt[HPT];
         uint32_t hsum = 0;
-        __syncthreads();
 #pragma unroll
         for (int q = 0; q < HPT; q++) {
             const int e = tid * HPT + q;
+            const int k = e % BZ, j = (e / BZ) % BY, i = e / (BZ * BY);
             hcnt[q] = 0;
-            if (e < nhc) {
-                const int k = e % bz, j = (e / bz) % by, i = e / (bz * by);
-                const int c = halo_cell(i + 1, j + 1, k + 1, hy, hz);
+            if (e < NHC && i < bx && j < by && k < bz) {
+                const int c = halo_cell(i + 1, j + 1, k + 1, HY, HZ);
                 hcnt[q] = s_cstart[c + 1] - s_cstart[c];
             }
             hsum += hcnt[q];
         }
         uint32_t n_home;
-        uint32_t hex = block_scan<NW>(hsum, s_wsum, &n_home);
+        const uint32_t hex0 = block_scan<NW>(hsum, s_wsum, &n_home);
         const bool dense = total > (uint32_t)HCAP || n_home > (uint32_t)HOMECAP;   // uniform
 
         auto eval_pair = [&](uint32_t e) -> float4 {
@@ -193,82 +193,66 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             return make_float4(ux, uy, uz, F);
         };
 
-        // Per-lane stream state of the current task: row sums and queued
-        // pairs per row. The queue fill is the same in every lane of a wave.
-        float px[3], py[3], pz[3];
-        uint32_t c0 = 0, c1 = 0, c2 = 0, listed = 0, queued = 0;
-        auto reset_rows = [&]() {
-#pragma unroll
-            for (int r = 0; r < 3; r++) px[r] = py[r] = pz[r] = 0.0f;
-        };
+        // Per-lane stream state of the current task: its plane sum and its
+        // queued pairs (s_list). The queue fill is the same in every lane of a wave.
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        uint32_t listed = 0, queued = 0;
         auto flush = [&]() {
             wave_sync();
             for (uint32_t i = (uint32_t)lane; i < queued; i += 64) qr[i] = eval_pair(qs[i]);
             wave_sync();
-            uint32_t c = 0;
-            for (uint32_t k = 0; k < c0; k++, c++) {
+            for (uint32_t c = 0; c < listed; c++) {
                 const float4 q = qr[s_list[c][tid]];
-                px[0] = fmaf(q.x, q.w, px[0]); py[0] = fmaf(q.y, q.w, py[0]); pz[0] = fmaf(q.z, q.w, pz[0]);
+                px = fmaf(q.x, q.w, px);
+                py = fmaf(q.y, q.w, py);
+                pz = fmaf(q.z, q.w, pz);
             }
-            for (uint32_t k = 0; k < c1; k++, c++) {
-                const float4 q = qr[s_list[c][tid]];
-                px[1] = fmaf(q.x, q.w, px[1]); py[1] = fmaf(q.y, q.w, py[1]); pz[1] = fmaf(q.z, q.w, pz[1]);
-            }
-            for (uint32_t k = 0; k < c2; k++, c++) {
-                const float4 q = qr[s_list[c][tid]];
-                px[2] = fmaf(q.x, q.w, px[2]); py[2] = fmaf(q.y, q.w, py[2]); pz[2] = fmaf(q.z, q.w, pz[2]);
-            }
-            c0 = c1 = c2 = 0;
             queued = 0;
             listed = 0;
             wave_sync();
         };
-        // One lane's candidate stream: up to three LDS ranges (row 0, 1, 2) in
-        // order, tested against home bead hidx at (hx, hy_, hz_). Called by the
-        // whole wave; an inactive lane passes empty ranges.
+        // One lane's candidate stream: up to three LDS ranges (its rows) in
+        // order, tested against home bead hidx at (hpx, hpy, hpz). Called by
+        // the whole wave; an inactive lane passes empty ranges.
         auto stream = [&](uint32_t hidx, float hpx, float hpy, float hpz, uint32_t a0, uint32_t n0, uint32_t a1,
                           uint32_t n1, uint32_t a2, uint32_t n2) {
             const uint32_t n01 = n0 + n1, len = n01 + n2;
-            const uint32_t wlen = wave_max(len);
+            const uint32_t b1 = a1 - n0, b2 = a2 - n01;     // candidate q of row 1 / 2 at q + b1 / q + b2
+            const uint32_t hkey = hidx << 16;
+            const uint32_t wlen = __builtin_amdgcn_readfirstlane(wave_max(len));
             for (uint32_t k = 0; k < wlen; k += kUnroll) {
-                uint32_t idx[kUnroll], row[kUnroll];
+                uint32_t idx[kUnroll];
                 float4 pc[kUnroll];
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
                     const uint32_t q = k + j;
-                    row[j] = (q >= n0 ? 1u : 0u) + (q >= n01 ? 1u : 0u);
-                    const uint32_t at = q < n0 ? a0 + q : (q < n01 ? a1 + (q - n0) : a2 + (q - n01));
+                    const uint32_t at = q + (q < n0 ? a0 : (q < n01 ? b1 : b2));
                     idx[j] = q < len ? at : hidx;          // beyond the stream: the home bead, never accepted
                     pc[j] = s_pos[idx[j]];
                 }
-                bool ok[kUnroll];
                 unsigned long long bal[kUnroll];
                 uint32_t tot = 0;
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
                     const float dx = hpx - pc[j].x, dy = hpy - pc[j].y, dz = hpz - pc[j].z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    ok[j] = r2 < 1.0f && r2 >= kMinR2;
-                    bal[j] = __builtin_amdgcn_ballot_w64(ok[j]);
+                    bal[j] = __builtin_amdgcn_ballot_w64(r2 < 1.0f && r2 >= kMinR2);
                     tot += (uint32_t)__popcll(bal[j]);
                 }
                 if (tot == 0) continue;
-                if (queued + tot > (uint32_t)QCAP || __builtin_amdgcn_ballot_w64(listed + kUnroll > (uint32_t)kList))
-                    flush();
-                uint32_t at = queued;
+                if (__builtin_amdgcn_ballot_w64(listed + kUnroll > (uint32_t)kList)) flush();
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
-                    if (ok[j]) {
-                        const uint32_t qp = at + (uint32_t)__popcll(bal[j] & below);
-                        qs[qp] = idx[j] | (hidx << 16);
+                    const uint32_t nj = (uint32_t)__popcll(bal[j]);
+                    if (nj == 0) continue;
+                    if (queued + nj > (uint32_t)QCAP) flush();
+                    if ((bal[j] >> lane) & 1ull) {
+                        const uint32_t qp = queued + (uint32_t)__popcll(bal[j] & below);
+                        qs[qp] = idx[j] | hkey;
                         s_list[listed++][tid] = (uint8_t)qp;
-                        c0 += row[j] == 0u ? 1u : 0u;
-                        c1 += row[j] == 1u ? 1u : 0u;
-                        c2 += row[j] == 2u ? 1u : 0u;
                     }
-                    at += (uint32_t)__popcll(bal[j]);
+                    queued += nj;
                 }
-                queued = at;
             }
         };
         // The force of one home bead from its plane sums (+ bonded terms),
@@ -279,7 +263,7 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             if (valid) {
                 if (!moving) {
                     fx = fy = fz = 0.0f;
-                } else {
+                } else if (!((fbits(p.w) >> 27) & 1u)) {       // a polymer bead (monomers have no bonded terms)
                     double bx_ = 0.0, by_ = 0.0, bz_ = 0.0;
                     if (bonded_sum(S, fbits(v.w), bx_, by_, bz_)) {     // pair force + bonded terms, rounded once
                         fx = (float)((double)fx + bx_);
@@ -296,56 +280,63 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
         };
 
         if (!dense) {
-            // ---- stage the halo: bead q of halo cell c (binary search on the LDS starts)
-            for (uint32_t q = (uint32_t)tid; q < total; q += NT) {
-                int lo = 0, hi = hc - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_cstart[mid] <= q) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int c = lo;
-                const int k = c % hz, j = (c / hz) % hy, i = c / (hz * hy);
-                const uint32_t g = s_gstart[c] + (q - s_cstart[c]);
-                float4 p = gpos[g];
-                p.x = p.x + image_shift(x0 - 1 + i, L0, Lx);
-                p.y = p.y + image_shift(y0 - 1 + j, L1, Ly);
-                p.z = p.z + image_shift(z0 - 1 + k, L2, Lz);
-                s_pos[q] = p;
-                s_vel[q] = gvel[g];
-            }
-            // home bead tables
+            // ---- stage the halo: each thread copies its cells' beads, image shifts added
 #pragma unroll
-            for (int q = 0; q < HPT; q++) {
-                const int e = tid * HPT + q;
-                if (e < nhc) {
-                    const int k = e % bz, j = (e / bz) % by, i = e / (bz * by);
-                    const int c = halo_cell(i + 1, j + 1, k + 1, hy, hz);
-                    const uint32_t first = s_cstart[c];
-                    for (uint32_t r = 0; r < hcnt[q]; r++) {
-                        s_hidx[hex + r] = (uint16_t)(first + r);
-                        s_hcell[hex + r] = (uint16_t)c;
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid * CPT + q;
+                if (c < HC && cnt[q]) {
+                    const int k = c % HZ, j = (c / HZ) % HY, i = c / (HZ * HY);
+                    const float sx = image_shift(x0 - 1 + i, L0, Lx), sy = image_shift(y0 - 1 + j, L1, Ly),
+                                sz = image_shift(z0 - 1 + k, L2, Lz);
+                    const uint32_t g = s_gstart[c], d = s_cstart[c];
+                    for (uint32_t r = 0; r < cnt[q]; r++) {
+                        float4 p = gpos[g + r];
+                        const float4 v = gvel[g + r];
+                        p.x = p.x + sx;
+                        p.y = p.y + sy;
+                        p.z = p.z + sz;
+                        s_pos[d + r] = p;
+                        s_vel[d + r] = v;
                     }
                 }
-                hex += hcnt[q];
+            }
+            // home bead tables
+            {
+                uint32_t hex = hex0;
+#pragma unroll
+                for (int q = 0; q < HPT; q++) {
+                    const int e = tid * HPT + q;
+                    if (e < NHC && hcnt[q]) {
+                        const int k = e % BZ, j = (e / BZ) % BY, i = e / (BZ * BY);
+                        const int c = halo_cell(i + 1, j + 1, k + 1, HY, HZ);
+                        const uint32_t first = s_cstart[c];
+                        for (uint32_t r = 0; r < hcnt[q]; r++) {
+                            s_hidx[hex + r] = (uint16_t)(first + r);
+                            s_hcell[hex + r] = (uint16_t)c;
+                        }
+                    }
+                    hex += hcnt[q];
+                }
             }
             for (int i = tid; i < kBins; i += NT) s_bin[i] = 0u;
-            if (tid == 0) { s_misc[2] = 0u; s_misc[3] = 0u; }
+            if (tid == 0) s_misc[3] = 0u;
             for (uint32_t i = (uint32_t)tid; i < 9u * HOMECAP; i += NT) (&s_part[0][0][0])[i] = 0.0f;
             __syncthreads();
 
-            // The three row ranges of task (h, plane): cells of the halo row
+            // ---- task ranges (home bead h, plane): cells of the halo row
             // (hi + plane - 1, hj + r - 1) from z-cell hk - 1 to hk + 1, with
             // cells wholly beyond the cutoff skipped. The gap from the home
             // bead to a neighbour cell is the distance to the shared face,
-            // edge or corner (fractions in the home cell are exact in FP32);
-            // a cell is skipped only when its squared gap exceeds 1 by a
-            // margin far above the rounding of the pair test, so no candidate
-            // the test would accept is lost.
-            auto task_rows = [&](uint32_t h, int plane, float4 &hp, uint32_t (&ra)[3], uint32_t (&rn)[3]) -> bool {
-                const uint32_t hidx = s_hidx[h];
+            // edge or corner (fractions in the home cell are exact in FP32); a
+            // cell is skipped only when its squared gap exceeds 1 by a margin
+            // far above the rounding of the pair test, so no candidate the
+            // test would accept is lost. Counted into length bins.
+            auto task_rows = [&](uint32_t t, uint32_t &hidx, float4 &hp, uint32_t (&ra)[3], uint32_t (&rn)[3]) {
+                const uint32_t h = t / 3u;
+                const int plane = (int)(t - 3u * h);
                 const int c = s_hcell[h];
-                const int hk = c % hz, hj = (c / hz) % hy, hi = c / (hz * hy);
+                const int hk = c % HZ, hj = (c / HZ) % HY, hi = c / (HZ * HY);
+                hidx = s_hidx[h];
                 hp = s_pos[hidx];
                 const bool moving = !((W->stationary >> (fbits(hp.w) >> 28)) & 1u);
                 constexpr float kReach = 1.0f + 1e-5f;
@@ -359,21 +350,18 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
                     const float g = fmaf(gy, gy, gx2);
                     const bool rowon = moving && g < kReach, loon = rowon && g + glo < kReach,
                                hion = rowon && g + ghi < kReach;
-                    const int base = halo_cell(hi + plane - 1, hj + r - 1, 0, hy, hz);
-                    const uint32_t b = s_cstart[base + (loon ? hk - 1 : hk)];
-                    const uint32_t e = s_cstart[base + (hion ? hk + 2 : hk + 1)];
+                    const int base = c + (plane - 1) * (HY * HZ) + (r - 1) * HZ;
+                    const uint32_t b = s_cstart[base + (loon ? -1 : 0)];
+                    const uint32_t e = s_cstart[base + (hion ? 2 : 1)];
                     ra[r] = b;
                     rn[r] = rowon ? e - b : 0u;
                 }
-                return moving;
             };
-
-            // ---- task lengths, counting sort by length
             const uint32_t n_tasks_all = 3u * n_home;
             for (uint32_t t = (uint32_t)tid; t < n_tasks_all; t += NT) {
+                uint32_t hidx, ra[3], rn[3];
                 float4 hp;
-                uint32_t ra[3], rn[3];
-                task_rows(t / 3u, (int)(t % 3u), hp, ra, rn);
+                task_rows(t, hidx, hp, ra, rn);
                 const uint32_t len = rn[0] + rn[1] + rn[2];
                 if (len) atomicAdd(&s_bin[min(len, (uint32_t)kBins - 1u)], 1u);
             }
@@ -391,9 +379,9 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             }
             __syncthreads();
             for (uint32_t t = (uint32_t)tid; t < n_tasks_all; t += NT) {
+                uint32_t hidx, ra[3], rn[3];
                 float4 hp;
-                uint32_t ra[3], rn[3];
-                task_rows(t / 3u, (int)(t % 3u), hp, ra, rn);
+                task_rows(t, hidx, hp, ra, rn);
                 const uint32_t len = rn[0] + rn[1] + rn[2];
                 if (len) s_task[atomicAdd(&s_bin[min(len, (uint32_t)kBins - 1u)], 1u)] = (uint16_t)t;
             }
@@ -405,23 +393,23 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             for (;;) {
                 uint32_t ch = 0;
                 if (lane == 0) ch = atomicAdd(&s_misc[3], 1u);
-                ch = __shfl(ch, 0, 64);
+                ch = __builtin_amdgcn_readfirstlane(__shfl(ch, 0, 64));
                 if (ch >= n_chunks) break;
                 const uint32_t ti = ch * 64u + (uint32_t)lane;
                 const bool act = ti < n_tasks;
                 const uint32_t t = act ? s_task[ti] : 0u;
+                uint32_t hidx, ra[3], rn[3];
                 float4 hp;
-                uint32_t ra[3], rn[3];
-                task_rows(t / 3u, (int)(t % 3u), hp, ra, rn);
+                task_rows(t, hidx, hp, ra, rn);
                 if (!act) rn[0] = rn[1] = rn[2] = 0u;
-                reset_rows();
-                stream(s_hidx[t / 3u], hp.x, hp.y, hp.z, ra[0], rn[0], ra[1], rn[1], ra[2], rn[2]);
+                px = py = pz = 0.0f;
+                stream(hidx, hp.x, hp.y, hp.z, ra[0], rn[0], ra[1], rn[1], ra[2], rn[2]);
                 if (queued) flush();
                 if (act) {
-                    const uint32_t h = t / 3u, plane = t % 3u;
-                    s_part[plane][0][h] = (px[0] + px[1]) + px[2];
-                    s_part[plane][1][h] = (py[0] + py[1]) + py[2];
-                    s_part[plane][2][h] = (pz[0] + pz[1]) + pz[2];
+                    const uint32_t h = t / 3u, plane = t - 3u * h;
+                    s_part[plane][0][h] = px;
+                    s_part[plane][1][h] = py;
+                    s_part[plane][2][h] = pz;
                 }
             }
             __syncthreads();
@@ -446,6 +434,7 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
             // ---- dense brick: home cell by home cell, up to NT home beads at
             // a time in LDS [0, NT), each stencil row staged in chunks at [NT, HCAP)
             constexpr uint32_t kChunk = (uint32_t)(HCAP - kDense);
+            const int nhc = bx * by * bz;
             for (int e = 0; e < nhc; e++) {
                 const int ek = e % bz, ej = (e / bz) % by, ei = e / (bz * by);
                 const int cx = x0 + ei, cy = y0 + ej, cz = z0 + ek;
@@ -461,7 +450,7 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
                     const bool moving = valid && !((W->stationary >> (fbits(hp.w) >> 28)) & 1u);
                     float sx_[3], sy_[3], sz_[3];
                     for (int plane = 0; plane < 3; plane++) {
-                        reset_rows();
+                        px = py = pz = 0.0f;
                         const int gxr = cx + plane - 1, gx = wrap(gxr, L0);
                         const float shx = image_shift(gxr, L0, Lx);
                         for (int r = 0; r < 3; r++) {
@@ -502,9 +491,9 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
                                 if (queued) flush();
                             }
                         }
-                        sx_[plane] = (px[0] + px[1]) + px[2];
-                        sy_[plane] = (py[0] + py[1]) + py[2];
-                        sz_[plane] = (pz[0] + pz[1]) + pz[2];
+                        sx_[plane] = px;
+                        sy_[plane] = py;
+                        sz_[plane] = pz;
                     }
                     finish_bead(valid, slot, hp, hv, moving, (sx_[0] + sx_[1]) + sx_[2], (sy_[0] + sy_[1]) + sy_[2],
                                 (sz_[0] + sz_[1]) + sz_[2]);

@@ -427,8 +427,8 @@ __device__ __forceinline__ void bonded_terms(const DeviceState &S, const float4 
 // queue fills, all lanes evaluate queued pairs side by side (64 per round,
 // whatever bead they belong to) and store each pair's unit vector and force
 // magnitude; each home bead then accumulates its own pairs in acceptance
-// order. So every force is summed in one fixed order: per row in candidate
-// order, rows as ((r0 + r1) + r2), planes as ((p0 + p1) + p2), independent of
+// order. So every force is summed in one fixed order: per plane one running
+// sum over its rows' candidates in order, planes as ((p0 + p1) + p2), independent of
 // batch, launch shape and queue flushes. kScanWidth (candidate loads in flight
 // per lane) and the occupancy target trade registers for memory-level
 // parallelism; every variant gives bit-identical forces.
@@ -564,14 +564,13 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
             uint32_t *__restrict__ qslot = s_qslot[plane];
             uint8_t *__restrict__ qhome = s_qhome[plane];
             float4 *__restrict__ qres = s_qres[plane];
-            float px[3] = {0.0f, 0.0f, 0.0f}, py[3] = {0.0f, 0.0f, 0.0f}, pz[3] = {0.0f, 0.0f, 0.0f};
-            uint32_t cnt[3] = {0u, 0u, 0u};     // this lane's queued pairs per row
-            uint32_t listed = 0;                // and in all (its list length)
+            float px = 0.0f, py = 0.0f, pz = 0.0f;  // this plane's sum
+            uint32_t listed = 0;                // this lane's queued pairs (its list length)
             uint32_t queued = 0;                // queue fill, the same in every lane
 
             // Evaluate every queued pair (dpd_maths_core_half_step_raw.hpp:129-216
             // with the terms grouped as a wr + sgw (scale u - sgw rdotv)), then add
-            // each lane's own pairs to its row sums in acceptance order.
+            // each lane's own pairs to its plane sum in acceptance order.
             auto flush = [&]() {
                 wave_sync();
                 for (uint32_t i = (uint32_t)lane; i < queued; i += kForceBeads) {
@@ -613,16 +612,11 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                     qres[i] = make_float4(ex, ey, ez, F);
                 }
                 wave_sync();
-                uint32_t c = 0;
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    for (uint32_t k = 0; k < cnt[r]; k++, c++) {
-                        const float4 q = qres[s_list[c][tid]];
-                        px[r] = fmaf(q.x, q.w, px[r]);
-                        py[r] = fmaf(q.y, q.w, py[r]);
-                        pz[r] = fmaf(q.z, q.w, pz[r]);
-                    }
-                    cnt[r] = 0;
+                for (uint32_t c = 0; c < listed; c++) {
+                    const float4 q = qres[s_list[c][tid]];
+                    px = fmaf(q.x, q.w, px);
+                    py = fmaf(q.y, q.w, py);
+                    pz = fmaf(q.z, q.w, pz);
                 }
                 queued = 0;
                 listed = 0;
@@ -675,7 +669,6 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                             qslot[qp] = slot[j] | cxy[r] | (k + j < na[r] ? zcA : zcB);
                             qhome[qp] = (uint8_t)lane;
                             s_list[listed++][tid] = (uint8_t)qp;
-                            cnt[r]++;
                         }
                         at += (uint32_t)__popcll(bal[j]);
                     }
@@ -683,9 +676,9 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 }
             }
             if (queued) flush();
-            fx = (px[0] + px[1]) + px[2];
-            fy = (py[0] + py[1]) + py[2];
-            fz = (pz[0] + pz[1]) + pz[2];
+            fx = px;
+            fy = py;
+            fz = pz;
         }
         if (!kBondWave && plane == 1) bonded_terms(S, vel, s, moving, lane, s_bond, s_nterms);
         // combine the three planes in order: (p0 + p1) + p2

@@ -100,6 +100,13 @@ inline void resolve_waves()
             int src = lane;
             if (op == 0) { f.res = ballot; f.wait = 0; continue; }
             if (op == 4) { f.res = 0; f.wait = 0; continue; }
+            if (op == 5) {                                                 // readfirstlane
+                int first = w0;
+                while (s.fibers[first].done) first++;
+                f.res = s.fibers[first].val;
+                f.wait = 0;
+                continue;
+            }
             if (op == 1) src = lane - f.arg < 0 ? lane : lane - f.arg;     // shfl_up
             if (op == 2) src = f.arg & 63;                                 // shfl
             if (op == 3) src = lane ^ f.arg;                               // shfl_xor
@@ -174,6 +181,7 @@ inline unsigned long long __builtin_amdgcn_ballot_w64(bool p) { return __ballot(
 inline uint32_t __shfl_up(uint32_t v, unsigned d, int = 64) { return (uint32_t)emu::wave_op(1, v, (int)d); }
 inline uint32_t __shfl(uint32_t v, int src, int = 64) { return (uint32_t)emu::wave_op(2, v, src); }
 inline uint32_t __shfl_xor(uint32_t v, int m, int = 64) { return (uint32_t)emu::wave_op(3, v, m); }
+inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t v) { return (uint32_t)emu::wave_op(5, v, 0); }
 inline void __builtin_amdgcn_s_sleep(int) { emu::yield_(); }
 inline void __builtin_amdgcn_wave_barrier() { (void)emu::wave_op(4, 0, 0); }
 inline void __builtin_amdgcn_fence(int, const char *) {}   // fibers run one at a time

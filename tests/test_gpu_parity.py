@@ -234,7 +234,12 @@ def test_autotune_leaves_state_unchanged(gpu_lib):
                                  dict(scan_lookback_tiles=0), dict(scan_lookback_tiles=0, graph_steps=4),
                                  dict(force_variant=1), dict(force_variant=2), dict(force_variant=3),
                                  dict(fused_integrate=0), dict(fused_integrate=1, graph_steps=6),
-                                 dict(fused_integrate=1, graph_steps=0, force_grid=64)])
+                                 dict(fused_integrate=1, graph_steps=0, force_grid=64),
+                                 dict(force_kernel=1), dict(force_kernel=1, force_brick=1),
+                                 dict(force_kernel=1, force_brick=2), dict(force_kernel=1, force_brick=3),
+                                 dict(force_kernel=1, force_brick=4), dict(force_kernel=1, force_brick=5),
+                                 dict(force_kernel=1, force_grid=64, graph_steps=6),
+                                 dict(force_kernel=1, fused_integrate=0)])
 def test_launch_shapes_bitwise_identical(gpu_lib, cfg):
     w = W.make_world((12, 10, 9), bonded_fraction=0.1, seed=41, a_matrix=W.random_a_matrix(np.random.default_rng(4)))
     _say(f"launch shape {cfg}")
@@ -291,7 +296,8 @@ def test_export_round_trip_and_momentum(gpu_lib):
 def test_dense_cell_matches_oracle(gpu_lib):
     """1500 beads in one cell (the reference AVX2 engine would throw on more
     than 32, dpd_engine_avx2_half_merge_tbb.hpp:816-820): the pair lists flush
-    as often as needed and the step still matches the oracle bit for bit."""
+    as often as needed, the brick kernel stages the rows in chunks, and the
+    step still matches the oracle bit for bit."""
     b = W.WorldBuilder((4, 4, 4), dt=0.001, seed=1)
     b.add_bead_type("A")
     b.set_interaction("A", "A", 25.0, 4.5)
@@ -300,13 +306,14 @@ def test_dense_cell_matches_oracle(gpu_lib):
     for i in range(1500):
         b.add_polymer(m, 1.0 + 0.9 * r.random(3))
     w = b.build()
-    _say("dense cell: 1500 beads in one cell")
-    ctx, slot = run_gpu(w, 1)
-    x, ww, f, t = ctx.download_f32(slot)
-    ctx.close()
     ref = O.F32State(w).step(1)
-    assert np.array_equal(x.view(np.uint32), ref.x.view(np.uint32))
-    assert np.array_equal(f.view(np.uint32), ref.f.view(np.uint32))
+    for kernel in (0, 1):       # per-slot kernel; brick kernel (its path for bricks beyond the LDS capacity)
+        _say(f"dense cell: 1500 beads in one cell, force_kernel={kernel}")
+        ctx, slot = run_gpu(w, 1, force_kernel=kernel)
+        x, ww, f, t = ctx.download_f32(slot)
+        ctx.close()
+        assert np.array_equal(x.view(np.uint32), ref.x.view(np.uint32))
+        assert np.array_equal(f.view(np.uint32), ref.f.view(np.uint32))
 
 
 def test_stationary_beads_stay_put(gpu_lib):
