@@ -242,6 +242,33 @@ typedef struct pdpd_bond_record {
 int pdpd_trace_forces(pdpd_ctx *ctx, int slot, pdpd_pair_record *pairs, uint32_t pair_capacity,
                       uint32_t *n_pairs, pdpd_bond_record *bonds, uint32_t bond_capacity, uint32_t *n_bonds);
 
+/* FP64 observables of one world, reduced on the device from its FP32 state
+ * (no download of the state): the momentum and sum v.v the reference checks
+ * (test_ordered_mesh.hpp:85-93; stats_dpd_engine_half_step.hpp:539-572 reports
+ * per-step velocity statistics), and the conservative virial. v is the
+ * exported velocity w + f dt/2; pairs are the unordered pairs of beads closer
+ * than the cutoff (nearest image, FP64 distance of the FP32 positions), each
+ * contributing r_ij . F^C_ij = a_ij (1 - r) r with the world's FP32 a_ij. The
+ * reference has no pressure; the one here is the textbook virial form with
+ * m = kT = 1. */
+typedef struct pdpd_observables {
+    int32_t slot;                /* world slot                                           */
+    uint32_t n_beads;
+    int64_t t;                   /* WorldState::t of the state the sums are taken from   */
+    double momentum[3];          /* sum_i v_i                                            */
+    double kinetic;              /* sum_i v_i . v_i                                      */
+    double virial;               /* sum over pairs with r < 1 of a_ij (1 - r) r          */
+    uint64_t n_pairs;            /* unordered pairs with r < 1                           */
+    double temperature;          /* kinetic / (3 n_beads)                                */
+    double pressure;             /* (n_beads temperature + virial / 3) / volume          */
+} pdpd_observables;
+
+/* Observables of every resident world (layout order, at most `capacity`; *n_out
+ * receives the count). Waits for the enqueued steps. */
+int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity, uint32_t *n_out);
+/* Observables of one world. */
+int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out);
+
 /* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
  * the worlds do not advance); the choice stays in the configuration. */

@@ -161,6 +161,10 @@ struct pdpd_ctx {
     DevBuf<NextBuffers> next;             // [buffer * 2 + counter copy]: what a chained step's force kernel writes
     DevBuf<BrickRec> bricks;              // pair-force bricks of the resident worlds
     uint32_t n_bricks = 0;
+    // observables: first bead block of every world (n_worlds + 1), per-block
+    // partial sums and per-world results (kObsValues doubles each)
+    DevBuf<uint32_t> obs_first;
+    DevBuf<double> obs_partial, obs_out;
 
     // interval snapshot: pos, vel, frc of every resident world (and the flags
     // word) copied on the device behind the enqueued steps; a world's share
@@ -217,6 +221,7 @@ struct pdpd_ctx {
         cell_par.release();
         next.release();
         bricks.release();
+        obs_first.release(); obs_partial.release(); obs_out.release();
         snap.release(); snap_flags.release();
         if (snap_host) (void)hipHostFree(snap_host);
         if (snap_event) (void)hipEventDestroy(snap_event);
@@ -474,6 +479,13 @@ int relayout(pdpd_ctx *ctx)
     HIP_TRY(dev_upload(ctx->bblocks, bb));
     HIP_TRY(dev_upload(ctx->fblocks, fb));
     if (int rc = upload_bricks(ctx, ws)) return rc;
+    {   // first bead block of each world (the blocks are world-contiguous)
+        std::vector<uint32_t> first(order.size() + 1, (uint32_t)bb.size());
+        for (size_t b = bb.size(); b-- > 0;) first[(size_t)bb[b].world] = (uint32_t)b;
+        for (size_t i = order.size(); i-- > 0;)
+            if (first[i] > first[i + 1]) first[i] = first[i + 1];     // a world without beads has no blocks
+        HIP_TRY(dev_upload(ctx->obs_first, first));
+    }
 
     // bonded terms with global bead indices; per bead (global index) its terms
     // in the order the per-polymer loop applies them: its bonds, then its
@@ -1388,6 +1400,70 @@ int pdpd_trace_forces(pdpd_ctx *ctx, int slot, pdpd_pair_record *pairs, uint32_t
         return fail(PDPD_ERR_INVALID, "force trace: record capacity too small");
     }
     return PDPD_OK;
+}
+
+// Per-world observables of the current state into ctx->obs_out (device),
+// stream-ordered. partial / out: device buffers of n_bblocks and n_worlds
+// entries of kObsValues doubles.
+static int enqueue_observables(pdpd_ctx *ctx, double *partial, double *out)
+{
+    HIP_TRY(launch_observables(ctx->state(), ctx->cur, ctx->n_bblocks, ctx->obs_first.p, ctx->n_worlds, partial, out,
+                               ctx->stream));
+    return PDPD_OK;
+}
+
+static void fill_observables(const pdpd_ctx *ctx, const WorldHost &w, const double *r, pdpd_observables *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->slot = w.slot;
+    o->n_beads = w.n;
+    o->t = w.t;
+    for (int k = 0; k < 3; k++) o->momentum[k] = r[k];
+    o->kinetic = r[3];
+    o->virial = r[4];
+    o->n_pairs = (uint64_t)r[5];
+    const double vol = (double)w.L[0] * (double)w.L[1] * (double)w.L[2];
+    o->temperature = w.n ? r[3] / (3.0 * (double)w.n) : 0.0;
+    o->pressure = ((double)w.n * o->temperature + r[4] / 3.0) / vol;
+    (void)ctx;
+}
+
+int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity, uint32_t *n_out)
+{
+    if (!ctx || (capacity && !out)) return fail(PDPD_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = ensure_layout(ctx);
+    if (rc) return rc;
+    if (n_out) *n_out = ctx->n_worlds;
+    if (ctx->n_worlds == 0) return PDPD_OK;
+    if ((rc = pdpd_sync(ctx))) return rc;
+    const size_t np = std::max<size_t>(1, ctx->n_bblocks) * kObsValues, no = (size_t)ctx->n_worlds * kObsValues;
+    if (ctx->obs_partial.n < np) HIP_TRY(dev_alloc(ctx->obs_partial, np));
+    if (ctx->obs_out.n < no) HIP_TRY(dev_alloc(ctx->obs_out, no));
+    if ((rc = enqueue_observables(ctx, ctx->obs_partial.p, ctx->obs_out.p))) return rc;
+    std::vector<double> r(no);
+    HIP_TRY(hipMemcpyAsync(r.data(), ctx->obs_out.p, no * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint32_t n = std::min(capacity, ctx->n_worlds);
+    for (uint32_t i = 0; i < n; i++) fill_observables(ctx, ctx->worlds.at(ctx->order[i]), &r[(size_t)i * kObsValues], &out[i]);
+    return PDPD_OK;
+}
+
+int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out)
+{
+    if (!ctx || !out) return fail(PDPD_ERR_INVALID, "null argument");
+    if (!find_world(ctx, slot)) return fail(PDPD_ERR_INVALID, "unknown slot");
+    int rc = ensure_layout(ctx);
+    if (rc) return rc;
+    std::vector<pdpd_observables> all(ctx->n_worlds);
+    uint32_t n = 0;
+    if ((rc = pdpd_get_observables(ctx, all.data(), (uint32_t)all.size(), &n))) return rc;
+    for (uint32_t i = 0; i < n; i++)
+        if (all[i].slot == slot) {
+            *out = all[i];
+            return PDPD_OK;
+        }
+    return fail(PDPD_ERR_STATE, "world not in the device layout");
 }
 
 // Times launch shapes that do not change results (the pair-force kernel,

@@ -58,6 +58,13 @@ hipError_t launch_trace_pairs(const DeviceState &S, int buf, uint32_t world, uin
 // slot_of_bead of a world moved from bead offset old_off to new_off (old = null: identity).
 hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
                                hipStream_t st);
+// Per-world FP64 observables of buffer buf (momentum, sum v.v, conservative
+// virial, pair and bead counts; kObsValues doubles per world) into out, via
+// one partial per bead block in partial (n_bead_blocks x kObsValues doubles).
+// first: n_worlds + 1 first bead-block indices.
+constexpr int kObsValues = 8;
+hipError_t launch_observables(const DeviceState &S, int buf, uint32_t n_bead_blocks, const uint32_t *first,
+                              uint32_t n_worlds, double *partial, double *out, hipStream_t st);
 hipError_t launch_debug_rng(uint32_t n, const uint64_t *th, const uint32_t *a, const uint32_t *b, uint32_t *u32,
                             float *f32, hipStream_t st);
 

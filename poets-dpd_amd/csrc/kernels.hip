@@ -812,6 +812,107 @@ __global__ __launch_bounds__(kBeadThreads) void k_stationary(DeviceState S, int 
     S.vel[nxt][slot] = make_float4(0.0f, 0.0f, 0.0f, v.w);
 }
 
+// -------------------------------------------------------- observables ----
+
+// Per-world sums in FP64 from the FP32 state (the reference's checks:
+// momentum sum_i v_i and sum_i v_i.v_i of test_ordered_mesh.hpp:85-93, the
+// per-step velocity statistics of stats_dpd_engine_half_step.hpp:539-572),
+// plus the conservative virial sum over unordered pairs closer than the
+// cutoff of r_ij . F^C_ij = a_ij (1 - r) r. v is the exported velocity
+// w + f dt/2 (FP64, as the export computes it); r is the nearest-image
+// distance of the FP32 positions in FP64 (exact differences), a_ij the world's
+// FP32 strength. Each bead block (256 beads of one world) writes its partial
+// sums to partial[block]; the world's blocks are then reduced in a fixed tree,
+// so the result depends only on the state and the block layout.
+constexpr int kObsThreads = kBeadThreads;
+
+__device__ __forceinline__ void block_sum_f64(double *v, double (*s_red)[kObsThreads], int tid)
+{
+#pragma unroll
+    for (int k = 0; k < kObsValues; k++) s_red[k][tid] = v[k];
+    __syncthreads();
+    for (int o = kObsThreads / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+#pragma unroll
+            for (int k = 0; k < kObsValues; k++) s_red[k][tid] += s_red[k][tid + o];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kObsValues; k++) v[k] = s_red[k][0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kObsThreads) void k_obs_partial(DeviceState S, int buf, double *__restrict__ partial)
+{
+    __shared__ double s_red[kObsValues][kObsThreads];
+    const BeadBlock blk = S.bead_blocks[blockIdx.x];
+    const WorldDev *__restrict__ W = &S.worlds[blk.world];
+    const int tid = threadIdx.x;
+    const uint32_t s = blk.begin + (uint32_t)tid;
+    double acc[kObsValues] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (s < blk.end) {
+        const float4 *__restrict__ pos = S.pos[buf];
+        const float4 *__restrict__ vel = S.vel[buf];
+        const float4 hp = pos[s], hv = vel[s], hf = S.frc[buf][s];
+        const double hdt = (double)W->dt * 0.5;
+        const double vx = (double)hv.x + (double)hf.x * hdt, vy = (double)hv.y + (double)hf.y * hdt,
+                     vz = (double)hv.z + (double)hf.z * hdt;
+        acc[0] = vx;
+        acc[1] = vy;
+        acc[2] = vz;
+        acc[3] = vx * vx + vy * vy + vz * vz;
+        acc[6] = 1.0;
+        const uint32_t me = fbits(hv.w), ht = fbits(hp.w) >> 28;
+        const int L[3] = {W->L[0], W->L[1], W->L[2]};
+        const int c[3] = {cell_coord(hp.x, L[0]), cell_coord(hp.y, L[1]), cell_coord(hp.z, L[2])};
+        for (int ox = -1; ox <= 1; ox++)
+            for (int oy = -1; oy <= 1; oy++)
+                for (int oz = -1; oz <= 1; oz++) {
+                    const int o[3] = {ox, oy, oz};
+                    int n[3];
+                    double sh[3];
+                    for (int d = 0; d < 3; d++) {
+                        n[d] = c[d] + o[d];
+                        sh[d] = 0.0;
+                        if (n[d] < 0) { n[d] += L[d]; sh[d] = -(double)L[d]; }
+                        else if (n[d] >= L[d]) { n[d] -= L[d]; sh[d] = (double)L[d]; }
+                    }
+                    const uint32_t cell = W->cell_off + (uint32_t)((n[0] * L[1] + n[1]) * L[2] + n[2]);
+                    for (uint32_t j = S.cell_start[cell]; j < S.cell_start[cell + 1]; j++) {
+                        if (fbits(vel[j].w) <= me) continue;            // each unordered pair once
+                        const float4 op = pos[j];
+                        const double dx = (double)hp.x - ((double)op.x + sh[0]);
+                        const double dy = (double)hp.y - ((double)op.y + sh[1]);
+                        const double dz = (double)hp.z - ((double)op.z + sh[2]);
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (!(r < 1.0)) continue;
+                        const double a = (double)W->a[ht * kMaxTypes + (fbits(op.w) >> 28)];
+                        acc[4] += a * (1.0 - r) * r;
+                        acc[5] += 1.0;
+                    }
+                }
+    }
+    block_sum_f64(acc, s_red, tid);
+    if (tid < kObsValues) partial[(size_t)blockIdx.x * kObsValues + tid] = acc[tid];
+}
+
+// One workgroup per world: its blocks [first[w], first[w + 1]) summed in a
+// fixed order into out[w * kObsValues ...].
+__global__ __launch_bounds__(kObsThreads) void k_obs_final(const double *__restrict__ partial,
+                                                           const uint32_t *__restrict__ first,
+                                                           double *__restrict__ out)
+{
+    __shared__ double s_red[kObsValues][kObsThreads];
+    const uint32_t w = blockIdx.x, b0 = first[w], b1 = first[w + 1];
+    double acc[kObsValues] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = b0 + threadIdx.x; b < b1; b += kObsThreads)
+#pragma unroll
+        for (int k = 0; k < kObsValues; k++) acc[k] += partial[(size_t)b * kObsValues + k];
+    block_sum_f64(acc, s_red, threadIdx.x);
+    if (threadIdx.x < kObsValues) out[(size_t)w * kObsValues + threadIdx.x] = acc[threadIdx.x];
+}
+
 // ------------------------------------------------------------ utility ----
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
@@ -927,6 +1028,15 @@ hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStrea
 {
     if (n == 0) return hipSuccess;
     k_rebase_index<<<ceil_div(n, 256), 256, 0, st>>>(vel, n, delta);
+    return hipGetLastError();
+}
+
+hipError_t launch_observables(const DeviceState &S, int buf, uint32_t n_bead_blocks, const uint32_t *first,
+                              uint32_t n_worlds, double *partial, double *out, hipStream_t st)
+{
+    if (n_worlds == 0) return hipSuccess;
+    if (n_bead_blocks) k_obs_partial<<<n_bead_blocks, kObsThreads, 0, st>>>(S, buf, partial);
+    k_obs_final<<<n_worlds, kObsThreads, 0, st>>>(partial, first, out);
     return hipGetLastError();
 }
 

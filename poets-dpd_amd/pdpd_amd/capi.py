@@ -84,6 +84,17 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Observables(C.Structure):
+    _fields_ = [("slot", C.c_int32), ("n_beads", C.c_uint32), ("t", C.c_int64), ("momentum", C.c_double * 3),
+                ("kinetic", C.c_double), ("virial", C.c_double), ("n_pairs", C.c_uint64),
+                ("temperature", C.c_double), ("pressure", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["momentum"] = list(self.momentum)
+        return d
+
+
 INTERVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32,
                           C.c_void_p, C.c_void_p, C.c_void_p)
 
@@ -121,6 +132,8 @@ SIGNATURES = {
     "pdpd_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "pdpd_run_profiled": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pdpd_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_size_t]),
+    "pdpd_get_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pdpd_world_observables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Observables)]),
     "pdpd_debug_rng": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     "pdpd_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
@@ -383,6 +396,18 @@ class Context:
         buf = C.create_string_buffer(256)
         check(self.lib.pdpd_autotune(self.h, int(steps_per_trial), buf, 256))
         return buf.value.decode()
+
+    def observables(self, slot=None):
+        """FP64 observables reduced on the device (pdpd_world_observables /
+        pdpd_get_observables): a dict for one slot, or {slot: dict} for all."""
+        if slot is not None:
+            o = Observables()
+            check(self.lib.pdpd_world_observables(self.h, slot, C.byref(o)))
+            return o.as_dict()
+        arr = (Observables * max(1, len(self.worlds)))()
+        n = C.c_uint32()
+        check(self.lib.pdpd_get_observables(self.h, C.cast(arr, C.c_void_p), len(self.worlds), C.byref(n)))
+        return {arr[i].slot: arr[i].as_dict() for i in range(min(n.value, len(self.worlds)))}
 
 
 def suggested_parts(total_beads):

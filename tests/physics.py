@@ -12,18 +12,35 @@ def kinetic_temperature(v):
 
 
 def _pairs(x, box, rmax):
-    """All pairs closer than rmax under the minimum image (brute force, small N)."""
-    n = x.shape[0]
-    out_d, out_i, out_j = [], [], []
-    for i in range(n - 1):
-        d = x[i + 1:] - x[i]
-        d -= np.round(d / box) * box
-        r = np.sqrt((d ** 2).sum(axis=1))
-        m = r < rmax
-        out_d.append(d[m])
-        out_i.append(np.full(m.sum(), i))
-        out_j.append(np.nonzero(m)[0] + i + 1)
-    return np.concatenate(out_d), np.concatenate(out_i), np.concatenate(out_j)
+    """All pairs closer than rmax under the minimum image: (d = x_j - x_i nearest
+    image, i, j) with i < j. A periodic k-d tree finds the candidates; the
+    distances are recomputed here in FP64 from the positions."""
+    from scipy.spatial import cKDTree
+    x = np.asarray(x, np.float64)
+    box = np.asarray(box, np.float64)
+    xt = np.mod(x, box)
+    ij = cKDTree(xt, boxsize=box).query_pairs(r=rmax, output_type="ndarray")
+    if ij.size == 0:
+        return np.zeros((0, 3)), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    i, j = ij[:, 0], ij[:, 1]
+    d = x[j] - x[i]
+    d -= np.round(d / box) * box
+    r = np.sqrt((d ** 2).sum(axis=1))
+    m = r < rmax
+    return d[m], i[m], j[m]
+
+
+def virial_sums(x, v, box, a, types):
+    """The engine's observables restated in FP64 on a downloaded state: momentum
+    sum v, sum v.v, the conservative virial sum over pairs (r < 1) of
+    a_ij (1 - r) r (a_ij as the engine holds it, FP32) and the pair count."""
+    x = np.asarray(x, np.float64)
+    v = np.asarray(v, np.float64)
+    d, i, j = _pairs(x, box, 1.0)
+    r = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    a32 = np.asarray(a, np.float32).astype(np.float64)
+    vir = (a32[types[i], types[j]] * (1.0 - r) * r).sum()
+    return v.sum(axis=0), (v * v).sum(), vir, len(r)
 
 
 def conservative_pressure(x, v, box, a, types):

@@ -27,7 +27,7 @@ FAMILIES = ["TestIsolatedMovingBead", "TestMultipleMovingBeads", "TestCollidingB
 MIN_TESTS = {"scenarios": 200, "differential": 200}      # the registered suite holds 225 scenarios
 
 
-def _run(mode, family, limit_s=100.0):
+def _run(mode, family, limit_s=600.0):
     """Run one scenario family (name prefix up to '['), streaming each TAP line as
     it arrives so a long family shows progress. stderr is merged into stdout; a
     reader thread collects the lines and the child is killed at the deadline,

@@ -119,3 +119,19 @@ def test_f32_pair_forces_independent_of_threads():
         O.lib().orc_set_threads(O.default_threads())
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_physics_pair_search_matches_brute_force():
+    """The periodic k-d tree pair search of the statistics helpers finds exactly
+    the pairs a brute-force nearest-image scan finds."""
+    import physics as P
+    w = W.make_world((5, 6, 4), seed=13)
+    d, i, j = P._pairs(w.x, w.box, 1.0)
+    got = set(zip(i.tolist(), j.tolist()))
+    want = set()
+    for a in range(w.n_beads - 1):
+        dd = w.x[a + 1:] - w.x[a]
+        dd -= np.round(dd / w.box) * w.box
+        for b in np.nonzero(np.sqrt((dd ** 2).sum(axis=1)) < 1.0)[0]:
+            want.add((a, a + 1 + int(b)))
+    assert got == want
