@@ -269,6 +269,22 @@ int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity
 /* Observables of one world. */
 int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out);
 
+/* pdpd_get_observables without any allocation. No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * partial sums of the bead blocks go to a file-scope device array of 8192
+ * blocks x 8 doubles (512 KiB, held by every GPU the library runs on until the
+ * process ends); the results go to out_dev, device memory the caller passes,
+ * 8 doubles per resident world in layout order: momentum x, y, z, sum v.v,
+ * virial, pairs, beads, 0 (the pdpd_observables fields before the derived
+ * ones). Enqueued on the context's stream, not waited for. Fails with
+ * PDPD_ERR_CAPACITY when the resident worlds span more than 8192 bead blocks
+ * (2^21 beads), PDPD_ERR_STATE when the layout must be rebuilt first (a world
+ * was added or removed since the last step), PDPD_ERR_INVALID when out_worlds
+ * is below the resident world count. Calls on one GPU share the array: calls
+ * that overlap (other contexts' streams, other threads) race. */
+int pdpd_observables_noalloc(pdpd_ctx *ctx, double *out_dev, uint32_t out_worlds);
+
 /* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
  * the worlds do not advance); the choice stays in the configuration. */

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1447,6 +1448,52 @@ int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity
     const uint32_t n = std::min(capacity, ctx->n_worlds);
     for (uint32_t i = 0; i < n; i++) fill_observables(ctx, ctx->worlds.at(ctx->order[i]), &r[(size_t)i * kObsValues], &out[i]);
     return PDPD_OK;
+}
+
+// ---- observables without allocation: partial sums in a file-scope device array
+
+// Bead blocks (256 beads each) whose partial sums the array holds: contexts of
+// up to 2^21 resident beads (about 8192 x 256), 512 KiB per GPU.
+constexpr uint32_t kObsStaticBlocks = 8192;
+static __device__ __align__(256) double scratch[kObsStaticBlocks * kObsValues];
+
+// The array's device address on `device` (current), looked up once per device.
+static int scratch_address(int device, double **out)
+{
+    static std::mutex mtx;
+    static double *addr[64] = {};
+    if (device < 0 || device >= 64) return fail(PDPD_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PDPD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[device] = (double *)p;
+    }
+    *out = addr[device];
+    return PDPD_OK;
+}
+
+int pdpd_observables_noalloc(pdpd_ctx *ctx, double *out_dev, uint32_t out_worlds)
+{
+    if (!ctx || !out_dev) return fail(PDPD_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->dirty)
+        return fail(PDPD_ERR_STATE, "the device layout must be rebuilt first (run a step or call an allocating "
+                                    "entry point after adding or removing worlds)");
+    if (out_worlds < ctx->n_worlds)
+        return fail(PDPD_ERR_INVALID, "out_dev holds " + std::to_string(out_worlds) + " worlds, " +
+                                          std::to_string(ctx->n_worlds) + " are resident");
+    if (ctx->n_bblocks > kObsStaticBlocks)
+        return fail(PDPD_ERR_CAPACITY, "resident worlds need " + std::to_string(ctx->n_bblocks) +
+                                           " bead blocks of partial sums; the static array holds " +
+                                           std::to_string(kObsStaticBlocks) + " (" +
+                                           std::to_string(kObsStaticBlocks * kBeadThreads) + " beads)");
+    if (ctx->n_worlds == 0) return PDPD_OK;
+    double *partial = nullptr;
+    if (int rc = scratch_address(ctx->device, &partial)) return rc;
+    return enqueue_observables(ctx, partial, out_dev);
 }
 
 int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out)

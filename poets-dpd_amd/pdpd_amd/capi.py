@@ -134,6 +134,7 @@ SIGNATURES = {
     "pdpd_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_size_t]),
     "pdpd_get_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "pdpd_world_observables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Observables)]),
+    "pdpd_observables_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "pdpd_debug_rng": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     "pdpd_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),

@@ -20,6 +20,7 @@
 #define __launch_bounds__(...)
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #define HIP_SYMBOL(x) (&x)
+#define __align__(n) alignas(n)
 
 struct float2 { float x, y; };
 struct float3 { float x, y, z; };
@@ -221,6 +222,7 @@ inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.001f; return hipSuccess; }
 inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = malloc(n); return *p ? hipSuccess : 2; }
 inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
 #define hipEventDisableTiming 2u
