@@ -262,8 +262,18 @@ public:
         return pdpd_can_support(&b.desc);
     }
 
-    // Bonds use the minimum image, so any bond shorter than half the box works.
-    double GetMaxBondLength() const override { return 1000; }
+    // Bonds use the nearest image, which resolves a bond only while it is
+    // shorter than half the smallest side of the box: just below that for the
+    // attached world, and for the smallest box the engine supports (3) when
+    // none is attached. A longer bond raises PDPD_FLAG_BOND_STRETCH and the
+    // run fails, as the reference's engines throw when a bond snaps
+    // (naive_dpd_engine_half_step.hpp:435-437).
+    double GetMaxBondLength() const override
+    {
+        double side = 3.0;
+        if (m_state) side = std::min(m_state->box[0], std::min(m_state->box[1], m_state->box[2]));
+        return std::nextafter(0.5 * side, 0.0);
+    }
 
     void PrepareResources() override { m_ctx.get(); }
 

@@ -29,8 +29,9 @@ extern "C" {
 #endif
 
 /* 2: pdpd_world_desc gained cell_conservative / cell_dissipative;
- * 3: pdpd_server_config gained contexts. */
-#define PDPD_ABI_VERSION 3
+ * 3: pdpd_server_config gained contexts;
+ * 4: pdpd_request_result gained fail_flags (status 4: failed). */
+#define PDPD_ABI_VERSION 4
 
 enum {
     PDPD_OK = 0,
@@ -46,6 +47,15 @@ enum {
 #define PDPD_FLAG_BAD_POSITION   0x1u   /* non-finite x, or a jump of more than one box */
 #define PDPD_FLAG_SCAN_TIMEOUT   0x2u   /* the cell scan look-back ran out of spins      */
 #define PDPD_FLAG_DENSE_ROW      0x4u   /* reserved: no longer raised (rows of any length are handled) */
+/* Per-world failure flags, raised by a world's own beads, bonds and forces
+ * (the reference's engines throw or exit on the same conditions): */
+#define PDPD_FLAG_CELL_JUMP      0x8u   /* a bead moved more than one cell in a step
+                                           (dpd_engine_avx2_half_merge_tbb.hpp:800-811)  */
+#define PDPD_FLAG_BOND_STRETCH   0x10u  /* a Hookean or angle bond longer than the engine's
+                                           maximum bond length, half the smallest box side
+                                           (naive_dpd_engine_half_step.hpp:435-437)       */
+#define PDPD_FLAG_FORCE_BOUND    0x20u  /* a force component not finite                     */
+#define PDPD_FLAG_VELOCITY_BOUND 0x40u  /* a velocity component not finite                  */
 
 /* Text of the last error on this thread ("" if none). */
 const char *pdpd_last_error(void);
@@ -138,6 +148,13 @@ int pdpd_run(pdpd_ctx *ctx, uint32_t nsteps);
 
 /* Wait for enqueued work; reports kernel-raised flags as errors. */
 int pdpd_sync(pdpd_ctx *ctx);
+
+/* PDPD_FLAG_* bits raised by one world since it was added (waits for the
+ * enqueued steps). pdpd_sync and the downloads fail with PDPD_ERR_STATE when a
+ * resident world raises one, naming the world; the flags stay with the world,
+ * so a caller running many worlds (the serving loop) can retire the failing
+ * one and continue with the others, whose results do not depend on it. */
+int pdpd_world_flags(pdpd_ctx *ctx, int slot, uint32_t *flags);
 
 /* Synchronise and export one world in bead-index order (the end of Run:
  * dpd_engine_avx2_half_merge_tbb.hpp:537-557, v += f*dt/2 in FP64).
@@ -318,7 +335,9 @@ typedef struct pdpd_request_result {
     uint32_t n_beads;
     uint32_t steps;
     double arrival_s, admit_s, done_s, latency_s;   /* seconds from the start of pdpd_server_run */
-    int status;                                     /* 0 done, 1 queued, 2 resident, 3 cancelled  */
+    int status;                                     /* 0 done, 1 queued, 2 resident, 3 cancelled,
+                                                       4 failed (its world raised fail_flags)       */
+    uint32_t fail_flags;                            /* PDPD_FLAG_* of a failed request            */
 } pdpd_request_result;
 
 typedef struct pdpd_server_stats {
@@ -326,6 +345,7 @@ typedef struct pdpd_server_stats {
     double makespan_s, gpu_busy_s;                  /* gpu_busy_s: time with a request resident */
     double bead_steps, bead_steps_per_s;             /* sum of beads x steps over the makespan */
     double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
+    uint64_t failed;                                 /* requests retired because their world failed */
 } pdpd_server_stats;
 
 int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out);

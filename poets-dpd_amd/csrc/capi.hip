@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cfloat>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -99,6 +100,7 @@ struct WorldHost {
     uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t term_base = 0;                        // index of the world's first bonded term (bonds, then angles)
     int64_t t_at_layout = 0;                       // t when the current layout was built
+    uint32_t flags = 0;                            // PDPD_FLAG_* raised by this world so far
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
 };
 
@@ -148,6 +150,7 @@ struct pdpd_ctx {
     // device layout
     DevBuf<float4> pos[2], vel[2], frc[2];
     DevBuf<uint32_t> key, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
+    DevBuf<uint32_t> wflags;              // per world in layout order: PDPD_FLAG_* raised on the device
     DevBuf<unsigned long long> scan_tiles;
     DevBuf<uint32_t> tile_sum;
     DevBuf<WorldDev> wdev;
@@ -171,6 +174,7 @@ struct pdpd_ctx {
     // word) copied on the device behind the enqueued steps; a world's share
     // goes to pinned host memory on a second stream when it is downloaded
     struct SnapWorld {
+        uint32_t index;                  // position in the layout (its flag word)
         uint32_t bead_off, n;
         int64_t t;
         double dt;
@@ -216,6 +220,7 @@ struct pdpd_ctx {
         for (int b = 0; b < 2; b++) { pos[b].release(); vel[b].release(); frc[b].release(); }
         key.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
+        wflags.release();
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
@@ -237,7 +242,7 @@ struct pdpd_ctx {
         S.key = key.p; S.perm = perm.p; S.sorted_idx = sorted_idx.p;
         S.cell_count = cell_count.p; S.cell_start = cell_start.p;
         S.scan_tiles = scan_tiles.p; S.scan_ticket = scan_ticket.p;
-        S.slot_of_bead = slot_of_bead.p; S.flags = flags.p;
+        S.slot_of_bead = slot_of_bead.p; S.flags = flags.p; S.wflags = wflags.p;
         S.worlds = wdev.p; S.ctr = ctr.p; S.bead_blocks = bblocks.p; S.force_blocks = fblocks.p;
         S.term_start = term_start.p; S.bead_terms = bead_terms.p; S.terms = terms.p; S.term_frc = term_frc.p;
         S.stationary_beads = stationary.p;
@@ -345,12 +350,51 @@ int upload_bricks(pdpd_ctx *ctx, const std::vector<const WorldHost *> &ws)
     return PDPD_OK;
 }
 
+// Move the device's per-world flags into the worlds (the layout's order names
+// them; worlds removed since are skipped) and clear them. Returns the union.
+int harvest_world_flags(pdpd_ctx *ctx, uint32_t *any)
+{
+    *any = 0;
+    if (!ctx->wflags.p || ctx->order.empty()) return PDPD_OK;
+    std::vector<uint32_t> f(ctx->order.size());
+    HIP_TRY(hipMemcpyAsync(f.data(), ctx->wflags.p, f.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t u = 0;
+    for (size_t i = 0; i < f.size(); i++) {
+        if (!f[i]) continue;
+        u |= f[i];
+        auto it = ctx->worlds.find(ctx->order[i]);
+        if (it != ctx->worlds.end()) it->second.flags |= f[i];
+    }
+    if (u) HIP_TRY(launch_fill_u32(ctx->wflags.p, (uint32_t)f.size(), 0, ctx->stream));
+    *any = u;
+    return PDPD_OK;
+}
+
+std::string flag_text(uint32_t f)
+{
+    std::string s;
+    auto add = [&](uint32_t bit, const char *what) {
+        if (f & bit) s += (s.empty() ? "" : "; ") + std::string(what);
+    };
+    add(PDPD_FLAG_BAD_POSITION, "a bead left the box (non-finite position or a jump of more than one box length)");
+    add(PDPD_FLAG_CELL_JUMP, "a bead moved more than one cell in a time-step");
+    add(PDPD_FLAG_BOND_STRETCH, "a bond is longer than half the smallest box side (it has snapped)");
+    add(PDPD_FLAG_FORCE_BOUND, "a force is not finite");
+    add(PDPD_FLAG_VELOCITY_BOUND, "a velocity is not finite");
+    return s;
+}
+
 // Rebuild the device layout: resident worlds are moved to their new offsets,
 // pending worlds are uploaded (unsorted; the next step sorts them).
 int relayout(pdpd_ctx *ctx)
 {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    {
+        uint32_t any;
+        if (int rc = harvest_world_flags(ctx, &any)) return rc;
+    }
     std::vector<int> order;
     for (auto &kv : ctx->worlds) order.push_back(kv.first);
 
@@ -440,6 +484,8 @@ int relayout(pdpd_ctx *ctx)
         HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
     }
     if (!ctx->scan_ticket.p) HIP_TRY(dev_alloc(ctx->scan_ticket, 1));
+    HIP_TRY(dev_alloc(ctx->wflags, std::max<size_t>(1, order.size())));
+    HIP_TRY(launch_fill_u32(ctx->wflags.p, (uint32_t)std::max<size_t>(1, order.size()), 0, ctx->stream));
     HIP_TRY(launch_fill_u32(ctx->cell_count.p, (uint32_t)nc, 0, ctx->stream));
 
     // world table; per-cell strengths of the worlds that have them, concatenated
@@ -458,6 +504,13 @@ int relayout(pdpd_ctx *ctx)
         d.seed_lo = (uint32_t)w.seed; d.seed_hi = (uint32_t)(w.seed >> 32);
         d.n_types = w.n_types;
         d.stationary = w.stationary;
+        // failure bounds: velocities and forces must stay finite (the bounds of
+        // the reference's validator, dpd_state_validator.hpp:147-152, apply to
+        // the states its tools read, not to its engines' runs); bonds must be
+        // shorter than the nearest image resolves
+        d.max_v = FLT_MAX;
+        d.max_f = FLT_MAX;
+        d.max_bond = 0.5 * (double)std::min(w.L[0], std::min(w.L[1], w.L[2]));
         memcpy(d.a, w.a, sizeof(d.a));
         memcpy(d.sg, w.sg, sizeof(d.sg));
         if (!w.cell_par.empty()) {
@@ -532,7 +585,7 @@ int relayout(pdpd_ctx *ctx)
         for (int cp = 0; cp < 2; cp++) {
             NextBuffers &b = nbuf[c * 2 + cp];
             b.pos = ctx->pos[c].p; b.vel = ctx->vel[c].p; b.frc = ctx->frc[c].p;
-            b.key = ctx->key.p; b.cell_count = ctx->cell_count.p; b.flags = ctx->flags.p;
+            b.key = ctx->key.p; b.cell_count = ctx->cell_count.p; b.flags = ctx->flags.p; b.wflags = ctx->wflags.p;
             b.scan_tiles = ctx->scan_tiles.p; b.scan_ticket = ctx->scan_ticket.p;
             b.ctr = ctx->ctr.p + (cp ^ 1) * ctx->n_worlds;
             b.n_tiles = scan_tiles((uint32_t)nc);
@@ -678,14 +731,25 @@ int enqueue_steps(pdpd_ctx *ctx, uint32_t nsteps)
 
 int check_flags(pdpd_ctx *ctx)
 {
-    uint32_t f = 0;
+    uint32_t f = 0, any = 0;
     HIP_TRY(hipMemcpy(&f, ctx->flags.p, sizeof(f), hipMemcpyDeviceToHost));
-    if (!f) return PDPD_OK;
-    ctx->stats.flags |= f;
-    HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (f & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
-    return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
+    if (int rc = harvest_world_flags(ctx, &any)) return rc;
+    ctx->stats.flags |= f | any;
+    if (f) {
+        HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (f & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
+    }
+    // every resident world that has raised a flag (now or before) fails the call, named
+    std::string msg;
+    for (int slot : ctx->order) {
+        auto it = ctx->worlds.find(slot);
+        if (it == ctx->worlds.end() || !it->second.flags) continue;
+        msg += (msg.empty() ? "" : " | ") + std::string("world ") + std::to_string(slot) + ": " +
+               flag_text(it->second.flags);
+    }
+    if (!msg.empty()) return fail(PDPD_ERR_STATE, msg);
+    return PDPD_OK;
 }
 
 WorldHost *find_world(pdpd_ctx *ctx, int slot)
@@ -963,6 +1027,20 @@ int pdpd_sync(pdpd_ctx *ctx)
     return PDPD_OK;
 }
 
+int pdpd_world_flags(pdpd_ctx *ctx, int slot, uint32_t *flags)
+{
+    WorldHost *w = ctx ? find_world(ctx, slot) : nullptr;
+    if (!w || !flags) return fail(PDPD_ERR_INVALID, !w ? "unknown slot" : "null output");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (w->resident && !ctx->dirty) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        uint32_t any;
+        if (int rc = harvest_world_flags(ctx, &any)) return rc;
+    }
+    *flags = w->flags;
+    return PDPD_OK;
+}
+
 int pdpd_world_download(pdpd_ctx *ctx, int slot, double *x, double *v, double *f, int64_t *t)
 {
     WorldHost *w = ctx ? find_world(ctx, slot) : nullptr;
@@ -1005,7 +1083,10 @@ int pdpd_snapshot(pdpd_ctx *ctx)
         if (ctx->snap.p) HIP_TRY(hipEventSynchronize(ctx->snap_event));
         HIP_TRY(dev_alloc(ctx->snap, 3 * n));
     }
-    if (!ctx->snap_flags.p) HIP_TRY(dev_alloc(ctx->snap_flags, 1));
+    if (ctx->snap_flags.n < 1 + (size_t)ctx->n_worlds) {
+        if (ctx->snap_flags.p) HIP_TRY(hipEventSynchronize(ctx->snap_event));
+        HIP_TRY(dev_alloc(ctx->snap_flags, 1 + (size_t)ctx->n_worlds));
+    }
     const int c = ctx->cur;        // the buffer the enqueued steps end in
     if (n) {
         HIP_TRY(hipMemcpyAsync(ctx->snap.p, ctx->pos[c].p, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1018,12 +1099,18 @@ int pdpd_snapshot(pdpd_ctx *ctx)
         HIP_TRY(hipMemcpyAsync(ctx->snap_flags.p, ctx->flags.p, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     else
         HIP_TRY(launch_fill_u32(ctx->snap_flags.p, 1, 0, ctx->stream));
+    if (ctx->n_worlds && ctx->wflags.p)
+        HIP_TRY(hipMemcpyAsync(ctx->snap_flags.p + 1, ctx->wflags.p, ctx->n_worlds * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->snap_event, ctx->stream));
     ctx->snap_beads = (uint32_t)n;
     ctx->snap_worlds.clear();
-    for (const auto &kv : ctx->worlds)
-        if (kv.second.resident)
-            ctx->snap_worlds[kv.first] = {kv.second.bead_off, kv.second.n, kv.second.t, kv.second.dt};
+    for (uint32_t i = 0; i < ctx->order.size(); i++) {      // worlds removed since the layout was built are skipped
+        auto it = ctx->worlds.find(ctx->order[i]);
+        if (it == ctx->worlds.end() || !it->second.resident) continue;
+        const WorldHost &w = it->second;
+        ctx->snap_worlds[ctx->order[i]] = {i, w.bead_off, w.n, w.t, w.dt};
+    }
     return PDPD_OK;
 }
 
@@ -1046,13 +1133,17 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
     for (int a = 0; a < 3 && n; a++)
         HIP_TRY(hipMemcpyAsync(ctx->snap_host + a * n, ctx->snap.p + a * N + sw.bead_off, n * sizeof(float4),
                                hipMemcpyDeviceToHost, ctx->copy_stream));
-    HIP_TRY(hipMemcpyAsync(ctx->snap_host + 3 * n, ctx->snap_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+    uint32_t *fh = reinterpret_cast<uint32_t *>(ctx->snap_host + 3 * n);      // [context-wide, this world]
+    HIP_TRY(hipMemcpyAsync(fh, ctx->snap_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->copy_stream));
+    HIP_TRY(hipMemcpyAsync(fh + 1, ctx->snap_flags.p + 1 + sw.index, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->copy_stream));
     HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
-    uint32_t flags;
-    memcpy(&flags, ctx->snap_host + 3 * n, sizeof(flags));
-    if (flags & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
-    if (flags) return fail(PDPD_ERR_STATE, "a bead left the box (non-finite position or a jump of more than one box length)");
+    const uint32_t gflags = fh[0], wflags = fh[1];
+    if (gflags & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
+    if (wflags) {
+        if (WorldHost *w = find_world(ctx, slot)) w->flags |= wflags;
+        return fail(PDPD_ERR_STATE, "world " + std::to_string(slot) + ": " + flag_text(wflags));
+    }
     const float4 *P = ctx->snap_host, *V = ctx->snap_host + n, *F = ctx->snap_host + 2 * n;
     const double hdt = (double)(float)sw.dt * 0.5;
     for (uint32_t s = 0; s < n; s++) {

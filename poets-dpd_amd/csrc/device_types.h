@@ -32,6 +32,11 @@ struct WorldDev {
     uint32_t stationary;   // bit k set: bead type k is stationary (never moves, v = f = 0 after a step)
     uint32_t spatial;      // 1: strengths per cell in cell_par (a and sg below unused)
     uint32_t cell_par_off; // first entry of this world's per-cell strengths in cell_par
+    // failure bounds: |v| <= max_v and |f| <= max_f per component (FLT_MAX:
+    // finite), bonds shorter than max_bond (half the smallest side, the
+    // nearest image's limit)
+    float max_v, max_f;
+    double max_bond;
     float a[kMaxTypes * kMaxTypes];    // conservative a_ij
     float sg[kMaxTypes * kMaxTypes];   // sqrt(gamma_ij)
 };
@@ -97,7 +102,8 @@ struct DeviceState {
     unsigned long long *scan_tiles;   // look-back state per scan tile
     uint32_t *scan_ticket;
     uint32_t *slot_of_bead;    // bonded (and stationary) beads: slot in the buffer a step starts from
-    uint32_t *flags;
+    uint32_t *flags;       // context-wide flags (PDPD_FLAG_SCAN_TIMEOUT)
+    uint32_t *wflags;      // per world: PDPD_FLAG_* raised by its beads, bonds and forces
     WorldDev *worlds;
     WorldCtr *ctr;
     BeadBlock *bead_blocks;    // kBeadThreads beads per block
@@ -118,7 +124,7 @@ struct DeviceState {
 // and counter copy cp ^ 1. One entry per (c, cp), in device memory.
 struct NextBuffers {
     float4 *pos, *vel, *frc;
-    uint32_t *key, *cell_count, *flags;
+    uint32_t *key, *cell_count, *flags, *wflags;
     unsigned long long *scan_tiles;
     uint32_t *scan_ticket;
     const WorldCtr *ctr;

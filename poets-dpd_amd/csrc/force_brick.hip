@@ -272,10 +272,11 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
                     }
                 }
                 gfrc[slot] = make_float4(fx, fy, fz, 0.0f);
+                if (const uint32_t ff = force_flags(W, fx, fy, fz)) atomicOr(&S.wflags[br.world], ff);
             }
             if (next) {
                 const NextBuffers B = next_buffers(next);
-                advance_next(B, W, slot, valid, p, v, make_float4(fx, fy, fz, 0.0f));
+                advance_next(B, W, (uint32_t)br.world, slot, valid, p, v, make_float4(fx, fy, fz, 0.0f));
             }
         };
 

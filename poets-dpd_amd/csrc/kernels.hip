@@ -68,13 +68,15 @@ __global__ __launch_bounds__(kBeadThreads) void k_integrate(DeviceState S, int c
     const WorldDev *__restrict__ W = &S.worlds[blk.world];
     const bool active = S.ctr[blk.world].remaining > 0;   // S.ctr points at this step's counters
     float4 p = S.pos[cur][s];
+    uint32_t fl = 0;
     if (active) {
         float4 v = S.vel[cur][s];
-        drift_bead(W, p, v, S.frc[cur][s]);
+        fl = drift_bead(W, p, v, S.frc[cur][s]);
         S.pos[cur][s] = p;
         S.vel[cur][s] = v;
     }
-    count_bead(S.flags, S.cell_count, S.key, W, s, p, active, true);   // lanes past the block's end have returned
+    // lanes past the block's end have returned
+    count_bead(&S.wflags[blk.world], fl, S.cell_count, S.key, W, s, p, active, true);
 }
 
 // --------------------------------------------------------------- scan ----
@@ -297,6 +299,7 @@ __device__ __forceinline__ void eval_term(const DeviceState &S, int cur, uint32_
         const double dy = min_image((double)pb.y - (double)pa.y, Ly);
         const double dz = min_image((double)pb.z - (double)pa.z, Lz);
         const double r = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
+        if (!(r < S.worlds[w].max_bond)) atomicOr(&S.wflags[w], PDPD_FLAG_BOND_STRETCH);
         const double hf = R.kappa * (R.p0 - r);
         const double sc = hf / r;
         out[0] = dx * sc;
@@ -311,6 +314,7 @@ __device__ __forceinline__ void eval_term(const DeviceState &S, int cur, uint32_
                  d1z = min_image((double)pc.z - (double)pb.z, Lz);
     const double r01 = sqrt(fma(d0z, d0z, fma(d0y, d0y, d0x * d0x)));
     const double r12 = sqrt(fma(d1z, d1z, fma(d1y, d1y, d1x * d1x)));
+    if (!(r01 < S.worlds[w].max_bond && r12 < S.worlds[w].max_bond)) atomicOr(&S.wflags[w], PDPD_FLAG_BOND_STRETCH);
     const double mag = r01 * r12;
     double hx = 0, hy = 0, hz = 0, tx = 0, ty = 0, tz = 0;
     if (mag > 0.0001) {
@@ -698,12 +702,13 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                 fz = (float)((double)fz + s_bond[2][lane]);
             }
             frc[s] = make_float4(fx, fy, fz, 0.0f);
+            if (const uint32_t ff = force_flags(W, fx, fy, fz)) atomicOr(&S.wflags[blk.world], ff);
         }
         __syncthreads();
         if (next && plane == 0) {          // the whole wave, converged; f as stored above by this lane
             const NextBuffers B = next_buffers(next);          // (after its compiler barrier: reloaded here)
             const uint32_t s2 = blk.begin + lane;
-            advance_next(B, &S.worlds[s_world], s2, live, s_hp[lane], s_hv[lane],
+            advance_next(B, &S.worlds[s_world], s_world, s2, live, s_hp[lane], s_hv[lane],
                          live ? frc[s2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         }
     }

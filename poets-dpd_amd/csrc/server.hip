@@ -233,17 +233,32 @@ int pdpd_server_run(pdpd_server *s)
             s->stats.bead_steps += (double)r->res.n_beads * adv;
             const bool finished = r->done >= r->steps;
             bool cancel = false;
+            uint32_t failed = 0;
             if (s->cb || finished) {
                 const size_t n = r->res.n_beads;
                 r->x.resize(3 * n); r->v.resize(3 * n); r->f.resize(3 * n);
                 int64_t t = 0;
                 int rc = pdpd_world_snapshot_download(s->ctxs[r->part], r->slot, r->x.data(), r->v.data(),
                                                       r->f.data(), &t);
-                if (rc) return rc;
-                if (s->cb) cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
-                                          r->f.data()) == 0;
+                if (rc == PDPD_ERR_STATE) {
+                    // the request's own world failed: retire it alone; the
+                    // others do not depend on it and run on
+                    uint32_t wf = 0;
+                    if (pdpd_world_flags(s->ctxs[r->part], r->slot, &wf) == PDPD_OK && wf) failed = wf;
+                    else return rc;
+                } else if (rc) {
+                    return rc;
+                }
+                if (s->cb && !failed)
+                    cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
+                                   r->f.data()) == 0;
             }
-            if (finished || cancel) {
+            if (failed) {
+                r->x.clear(); r->v.clear(); r->f.clear();
+                r->res.fail_flags = failed;
+                s->stats.failed++;
+            }
+            if (finished || cancel || failed) {
                 int rc = pdpd_world_remove(s->ctxs[r->part], r->slot);
                 if (rc) return rc;
                 s->part_beads[r->part] -= r->res.n_beads;
@@ -251,9 +266,9 @@ int pdpd_server_run(pdpd_server *s)
                 r->inflight = 0;
                 r->res.done_s = rel();
                 r->res.latency_s = r->res.done_s - r->arrival * scale;
-                r->res.status = finished ? 0 : 3;   // done / cancelled
+                r->res.status = failed ? 4 : (finished ? 0 : 3);   // failed / done / cancelled
                 used -= r->bytes;
-                s->stats.completed++;
+                if (!failed) s->stats.completed++;
             }
         }
         std::vector<Request *> keep;
@@ -339,6 +354,7 @@ int pdpd_server_run(pdpd_server *s)
     std::vector<double> lat;
     for (auto &kv : s->reqs)
         if (kv.second->res.status == 0) lat.push_back(kv.second->res.latency_s);
+
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double p) {
         if (lat.empty()) return 0.0;

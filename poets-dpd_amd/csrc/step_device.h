@@ -100,9 +100,14 @@ __device__ __forceinline__ int cell_coord(float x, int L)
 // v(t) = w + f dt/2, v(t+dt/2) = v(t) + f dt/2, x(t+dt) = x + dt v(t+dt/2),
 // then w = v(t+dt/2). A bead of a stationary type keeps x; the pair force sees
 // v(t) (the imported v on the first step, 0 after) --
-// naive_dpd_engine_half_merge.hpp:327-331.
-__device__ __forceinline__ void drift_bead(const WorldDev *__restrict__ W, float4 &p, float4 &v, const float4 f)
+// naive_dpd_engine_half_merge.hpp:327-331. Returns the failure flags the move
+// raises: PDPD_FLAG_CELL_JUMP for a move of more than one cell on an axis
+// (the periodic wrap aside; dpd_engine_avx2_half_merge_tbb.hpp:800-811),
+// PDPD_FLAG_VELOCITY_BOUND for a velocity component beyond the world's bound
+// (by default: not finite).
+__device__ __forceinline__ uint32_t drift_bead(const WorldDev *__restrict__ W, float4 &p, float4 &v, const float4 f)
 {
+    uint32_t fl = 0;
     const float dt = W->dt, hdt = W->hdt;
     const float fv[3] = {f.x, f.y, f.z};
     float x[3] = {p.x, p.y, p.z};
@@ -119,12 +124,16 @@ __device__ __forceinline__ void drift_bead(const WorldDev *__restrict__ W, float
             float xn = fmaf(vh, dt, x[d]);          // x(t+dt)    = x + dt v(t+dt/2)
             xn += (xn < 0.0f) ? Lf : 0.0f;
             xn -= (xn >= Lf) ? Lf : 0.0f;
+            const int jump = abs(cell_coord(xn, W->L[d]) - cell_coord(x[d], W->L[d]));
+            fl |= (jump > 1 && jump < W->L[d] - 1) ? PDPD_FLAG_CELL_JUMP : 0u;
+            fl |= fabsf(vh) <= W->max_v ? 0u : PDPD_FLAG_VELOCITY_BOUND;
             x[d] = xn;
             wv[d] = vh;
         }
     }
     p = make_float4(x[0], x[1], x[2], p.w);
     v = make_float4(wv[0], wv[1], wv[2], v.w);
+    return fl;
 }
 
 // Runs of equal keys in consecutive lanes of one wave (called converged; the
@@ -147,17 +156,20 @@ __device__ __forceinline__ void key_runs(uint32_t key, bool valid, int &start, i
 // The cell key of the bead in slot s at position p (with kInactive for a
 // bead that does not move this step), and its count in cell_count: one
 // atomic per run of equal keys, whose result is not waited for. Raises
-// PDPD_FLAG_BAD_POSITION for a position outside the box. Called by all lanes
-// of a wave, converged.
-__device__ __forceinline__ void count_bead(uint32_t *__restrict__ flags, uint32_t *__restrict__ cell_count,
-                                           uint32_t *__restrict__ key_out, const WorldDev *__restrict__ W, uint32_t s,
-                                           const float4 p, bool active, bool valid)
+// PDPD_FLAG_BAD_POSITION for a position outside the box, with the flags
+// `more` of its move, in its world's flag word wflag. Called by all lanes of a
+// wave, converged.
+__device__ __forceinline__ void count_bead(uint32_t *__restrict__ wflag, uint32_t more,
+                                           uint32_t *__restrict__ cell_count, uint32_t *__restrict__ key_out,
+                                           const WorldDev *__restrict__ W, uint32_t s, const float4 p, bool active,
+                                           bool valid)
 {
     const float x[3] = {p.x, p.y, p.z};
     bool bad = false;
 #pragma unroll
     for (int d = 0; d < 3; d++) bad |= !(x[d] >= 0.0f && x[d] < W->Lf[d]);
-    if (valid && bad) atomicOr(flags, PDPD_FLAG_BAD_POSITION);
+    const uint32_t fl = more | (bad ? PDPD_FLAG_BAD_POSITION : 0u);
+    if (valid && fl) atomicOr(wflag, fl);
     const int c0 = cell_coord(x[0], W->L[0]), c1 = cell_coord(x[1], W->L[1]), c2 = cell_coord(x[2], W->L[2]);
     const uint32_t key = valid ? W->cell_off + (uint32_t)((c0 * W->L[1] + c1) * W->L[2] + c2) : 0xffffffffu;
     int start, next;
@@ -227,18 +239,27 @@ __device__ __forceinline__ uint32_t bonded_sum(const DeviceState &S, uint32_t b,
 // device memory (null: not chained), after its pair loop, so that their
 // addresses take no scalar registers during the loop.
 
-// One bead (all lanes of the wave call it, the valid ones a prefix).
-__device__ __forceinline__ void advance_next(const NextBuffers &B, const WorldDev *__restrict__ W, uint32_t s,
-                                             bool valid, float4 p, float4 v, const float4 f)
+// One bead of world `world` (all lanes of the wave call it, the valid ones a prefix).
+__device__ __forceinline__ void advance_next(const NextBuffers &B, const WorldDev *__restrict__ W, uint32_t world,
+                                             uint32_t s, bool valid, float4 p, float4 v, const float4 f)
 {
+    uint32_t fl = 0;
     if (valid) {
         // k_stationary zeroes the velocity of a stationary bead after this kernel
         if ((W->stationary >> (fbits(p.w) >> 28)) & 1u) v = make_float4(0.0f, 0.0f, 0.0f, v.w);
-        drift_bead(W, p, v, f);
+        fl = drift_bead(W, p, v, f);
         B.pos[s] = p;
         B.vel[s] = v;
     }
-    count_bead(B.flags, B.cell_count, B.key, W, s, p, true, valid);
+    count_bead(&B.wflags[world], fl, B.cell_count, B.key, W, s, p, true, valid);
+}
+
+// PDPD_FLAG_FORCE_BOUND for a force component beyond the world's bound (by
+// default: not finite).
+__device__ __forceinline__ uint32_t force_flags(const WorldDev *__restrict__ W, float fx, float fy, float fz)
+{
+    const float m = W->max_f;
+    return (fabsf(fx) <= m && fabsf(fy) <= m && fabsf(fz) <= m) ? 0u : PDPD_FLAG_FORCE_BOUND;
 }
 
 // The table entry, read after a compiler barrier so the loads stay here.

@@ -23,7 +23,7 @@ class PdpdError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 3          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
+ABI_VERSION = 4          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
 
 
 class WorldDesc(C.Structure):
@@ -71,14 +71,16 @@ class ServerConfig(C.Structure):
 
 class RequestResult(C.Structure):
     _fields_ = [("id", C.c_uint64), ("n_beads", C.c_uint32), ("steps", C.c_uint32), ("arrival_s", C.c_double),
-                ("admit_s", C.c_double), ("done_s", C.c_double), ("latency_s", C.c_double), ("status", C.c_int)]
+                ("admit_s", C.c_double), ("done_s", C.c_double), ("latency_s", C.c_double), ("status", C.c_int),
+                ("fail_flags", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("completed", C.c_uint64), ("intervals", C.c_uint64),
                 ("max_resident", C.c_uint64), ("makespan_s", C.c_double), ("gpu_busy_s", C.c_double),
                 ("bead_steps", C.c_double), ("bead_steps_per_s", C.c_double), ("p50_latency_s", C.c_double),
-                ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("max_latency_s", C.c_double)]
+                ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("max_latency_s", C.c_double),
+                ("failed", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -118,6 +120,7 @@ SIGNATURES = {
     "pdpd_world_set_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
     "pdpd_run": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pdpd_sync": (C.c_int, [C.c_void_p]),
+    "pdpd_world_flags": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32)]),
     "pdpd_world_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_int64)]),
     "pdpd_snapshot": (C.c_int, [C.c_void_p]),
@@ -323,6 +326,12 @@ class Context:
 
     def run_profiled(self, nsteps):
         check(self.lib.pdpd_run_profiled(self.h, int(nsteps)))
+
+    def flags(self, slot):
+        """PDPD_FLAG_* bits the world has raised (pdpd_world_flags)."""
+        f = C.c_uint32()
+        check(self.lib.pdpd_world_flags(self.h, slot, C.byref(f)))
+        return f.value
 
     def download(self, slot):
         n = self.worlds[slot][0]

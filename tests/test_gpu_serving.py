@@ -84,3 +84,54 @@ def test_callback_cancels_request(gpu_lib):
     r = srv.result(rid)
     assert r["status"] == 3
     srv.close()
+
+
+def test_failing_request_retired_alone(gpu_lib):
+    """A request whose world fails (one bead thrown out of the box: the
+    reference's engines throw on such a state, dpd_engine_avx2_half_merge_tbb.hpp:800-820)
+    is retired with its flags; the requests sharing its launches finish
+    bit-identical to their solo runs."""
+    bad = W.make_world((6, 5, 7), bonded_fraction=0.2, seed=700)
+    bad.v = bad.v.copy()
+    bad.v[3] = [1e30, 0.0, 0.0]
+    good = [W.make_world((7, 6, 5), bonded_fraction=0.2, seed=701), W.make_world((5, 6, 6), seed=702)]
+    srv = capi.Server(device=0, interval_steps=20, time_scale=0.0)
+    srv.set_callback(lambda rid, done, t, x, v, f: True)
+    ids = [srv.submit(good[0], 0.0, 60), srv.submit(bad, 0.0, 60), srv.submit(good[1], 0.0, 60)]
+    srv.run()
+    st = srv.stats()
+    rb = srv.result(ids[1], with_state=False)
+    assert st["failed"] == 1 and st["completed"] == 2
+    assert rb["status"] == 4 and rb["fail_flags"] & 0x1
+    for rid, w in ((ids[0], good[0]), (ids[2], good[1])):
+        r = srv.result(rid)
+        x, v, f, t = _solo(w, 60)
+        assert r["status"] == 0
+        assert np.array_equal(r["x"], x) and np.array_equal(r["v"], v) and np.array_equal(r["f"], f)
+    srv.close()
+
+
+def test_world_flags_name_the_failing_world(gpu_lib):
+    """pdpd_sync fails naming the world that raised a flag; a snapped bond (longer
+    than half the box, naive_dpd_engine_half_step.hpp:435-437) and a bead out of
+    the box are reported per world."""
+    b = W.WorldBuilder((4, 4, 4), dt=0.01, seed=1)
+    b.add_bead_type("A")
+    b.set_interaction("A", "A", 25.0, 4.5)
+    m = b.add_polymer_type("AA", ["A", "A"], bonds=[(0, 1, 128.0, 0.5)])
+    b.add_polymer(m, [[0.5, 0.5, 0.5], [2.6, 2.6, 0.5]])       # bond of 2.97 > half of 4
+    snapped = b.build()
+    ok = W.make_world((5, 5, 5), seed=3)
+    ctx = capi.Context(0)
+    s_ok, s_bad = ctx.add(ok), ctx.add(snapped)
+    ctx.run(2)
+    with pytest.raises(capi.PdpdError) as e:
+        ctx.sync()
+    assert f"world {s_bad}" in str(e.value) and "bond" in str(e.value) and f"world {s_ok}" not in str(e.value)
+    assert ctx.flags(s_bad) & 0x10 and ctx.flags(s_ok) == 0
+    ctx.remove(s_bad)
+    ctx.run(3)
+    ctx.sync()                                                  # the failing world gone, the run goes on
+    x, v, f, t = ctx.download(s_ok)
+    assert t == 5
+    ctx.close()

@@ -185,6 +185,25 @@ for k, (w, _) in enumerate(reqs):
     ok &= gt == t and np.array_equal(gx, x) and np.array_equal(gv, v) and np.array_equal(gf, f)
 print("ensemble", "ok" if ok else "WRONG", flush=True)
 ens.close()
+# a request whose world fails (one bead thrown out of the box) is retired alone;
+# the other two finish bit-identical to their solo runs
+bad = W.make_world((4, 5, 4), bonded_fraction=0.2, seed=700)
+bad.v = bad.v.copy()
+bad.v[3] = [1e30, 0.0, 0.0]
+good = [W.make_world((4, 4, 5), bonded_fraction=0.2, seed=701), W.make_world((5, 4, 4), seed=702)]
+srv = capi.Server(device=0, interval_steps=10, time_scale=0.0)
+srv.set_callback(lambda rid, done, t, x, v, f: True)
+ids = [srv.submit(good[0], 0.0, 30), srv.submit(bad, 0.0, 30), srv.submit(good[1], 0.0, 30)]
+srv.run()
+st = srv.stats()
+rb = srv.result(ids[1], with_state=False)
+ok = st["failed"] == 1 and st["completed"] == 2 and rb["status"] == 4 and (rb["fail_flags"] & 1) == 1
+for rid, w in ((ids[0], good[0]), (ids[2], good[1])):
+    r = srv.result(rid)
+    x, v, f, t = solo(w, 30)
+    ok &= r["status"] == 0 and np.array_equal(r["x"], x) and np.array_equal(r["v"], v) and np.array_equal(r["f"], f)
+print("failed_request", "ok" if ok else "WRONG", st, rb, flush=True)
+srv.close()
 """
 
 
@@ -192,8 +211,11 @@ def test_emulated_serving_pipeline(emu_build):
     """The serving loop with one interval in flight (snapshot, next interval,
     then downloads and callbacks) over two contexts: queueing under a memory
     budget, a callback per interval, a cancellation, and results equal to solo
-    runs bit for bit; an ensemble of worlds over three contexts likewise."""
+    runs bit for bit; an ensemble of worlds over three contexts likewise; a
+    request whose world fails (a bead thrown out of the box) retired alone,
+    with its flags, while the others finish bit-identical to their solo runs."""
     code = SERVE_PROBE % dict(pkg=os.path.join(ROOT, "poets-dpd_amd"), lib=os.path.join(emu_build, "libpdpd_emu.so"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "serving ok" in r.stdout and "ensemble ok" in r.stdout, r.stdout
+    assert "failed_request ok" in r.stdout, r.stdout
