@@ -246,7 +246,9 @@ def run_served(args, capi, W):
             "p99_latency_s": st["p99_latency_s"], "mean_latency_s": st["mean_latency_s"],
             "p50_admission_wait_s": pct(waits, 0.5), "p99_admission_wait_s": pct(waits, 0.99),
             "value": st["bead_steps_per_s"], "unit": "bead-steps/s", "makespan_s": st["makespan_s"],
-            "max_resident": st["max_resident"], "callbacks": seen[0], "interval_steps": 500,
+            "max_resident": st["max_resident"], "max_resident_beads": st["max_resident_beads"],
+            "relayouts": st["relayouts"], "relayout_mean_ms": 1e3 * st["relayout_mean_s"],
+            "relayout_max_ms": 1e3 * st["relayout_max_s"], "callbacks": seen[0], "interval_steps": 500,
             "arrival_rate_per_s": 50.0, "last_arrival_s": reqs[-1]["arrival"] if reqs else 0.0,
             "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
 

@@ -30,8 +30,11 @@ extern "C" {
 
 /* 2: pdpd_world_desc gained cell_conservative / cell_dissipative;
  * 3: pdpd_server_config gained contexts;
- * 4: pdpd_request_result gained fail_flags (status 4: failed). */
-#define PDPD_ABI_VERSION 4
+ * 4: pdpd_request_result gained fail_flags (status 4: failed);
+ * 5: pdpd_stats gained relayouts / relayout_s / relayout_max_s / relayout_max_beads,
+ *    pdpd_server_stats gained relayouts / relayout_mean_s / relayout_max_s /
+ *    max_resident_beads. */
+#define PDPD_ABI_VERSION 5
 
 enum {
     PDPD_OK = 0,
@@ -207,6 +210,10 @@ typedef struct pdpd_stats {
      * bonded terms, k_stationary the velocity clear of stationary beads (0 without them) */
     double k_integrate_ms, k_scan_ms, k_scatter_ms, k_reorder_ms, k_force_ms, k_stationary_ms;
     uint64_t profiled_steps;
+    uint64_t relayouts;           /* device layouts built (a world added or removed before a run) */
+    double relayout_s;            /* host time spent building them (the device work is enqueued)  */
+    double relayout_max_s;        /* the longest of them                                           */
+    uint64_t relayout_max_beads;  /* resident beads of the layout that took relayout_max_s         */
 } pdpd_stats;
 
 int pdpd_get_stats(pdpd_ctx *ctx, pdpd_stats *out);
@@ -346,6 +353,9 @@ typedef struct pdpd_server_stats {
     double bead_steps, bead_steps_per_s;             /* sum of beads x steps over the makespan */
     double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
     uint64_t failed;                                 /* requests retired because their world failed */
+    /* layouts rebuilt on admission or retirement (host time; the moves run on the device) */
+    uint64_t relayouts, max_resident_beads;
+    double relayout_mean_s, relayout_max_s;
 } pdpd_server_stats;
 
 int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out);

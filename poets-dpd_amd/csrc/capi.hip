@@ -99,6 +99,7 @@ struct WorldHost {
     std::vector<uint32_t> term_start_local, term_list_local;
     uint32_t bead_off = 0, cell_off = 0;          // layout
     uint32_t term_base = 0;                        // index of the world's first bonded term (bonds, then angles)
+    uint32_t list_base = 0;                        // index of the world's first per-bead term-list entry
     int64_t t_at_layout = 0;                       // t when the current layout was built
     uint32_t flags = 0;                            // PDPD_FLAG_* raised by this world so far
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
@@ -151,6 +152,19 @@ struct pdpd_ctx {
     DevBuf<float4> pos[2], vel[2], frc[2];
     DevBuf<uint32_t> key, perm, sorted_idx, cell_count, cell_start, slot_of_bead, flags, scan_ticket;
     DevBuf<uint32_t> wflags;              // per world in layout order: PDPD_FLAG_* raised on the device
+    // the previous layout's copies of the tables a relayout rebuilds on the
+    // device from the current ones (then swapped in)
+    DevBuf<uint32_t> slot_of_bead_alt, wflags_alt, term_start_alt, bead_terms_alt;
+    DevBuf<TermRec> terms_alt;
+    DevBuf<uint32_t> remap;               // relayout: new world index -> old
+    // buffers replaced while the stream may still read them, freed once it is idle
+    std::vector<void *> graveyard;
+    // pinned staging of a relayout's host-to-device uploads; reused once the
+    // previous relayout's copies are done (stage_event)
+    char *stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
+    hipEvent_t stage_event = nullptr;
+    bool stage_pending = false;
     DevBuf<unsigned long long> scan_tiles;
     DevBuf<uint32_t> tile_sum;
     DevBuf<WorldDev> wdev;
@@ -221,6 +235,11 @@ struct pdpd_ctx {
         key.release(); perm.release(); sorted_idx.release(); cell_count.release();
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
         wflags.release();
+        slot_of_bead_alt.release(); wflags_alt.release(); term_start_alt.release(); bead_terms_alt.release();
+        terms_alt.release(); remap.release();
+        for (void *q : graveyard) (void)hipFree(q);
+        if (stage) (void)hipHostFree(stage);
+        if (stage_event) (void)hipEventDestroy(stage_event);
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
@@ -385,122 +404,128 @@ std::string flag_text(uint32_t f)
     return s;
 }
 
+// Device buffer with room for n entries; contents are not kept. A replaced
+// buffer may still be read by enqueued work: it is freed when the stream is
+// next idle (free_graveyard), so growing never waits for the device.
+template <class T>
+int reserve(pdpd_ctx *ctx, DevBuf<T> &b, size_t n)
+{
+    if (n == 0) n = 1;
+    if (b.p && b.n >= n) return PDPD_OK;
+    const size_t cap = std::max(n, b.n + b.n / 2);
+    T *p = nullptr;
+    const hipError_t e = hipMalloc((void **)&p, cap * sizeof(T));
+    if (e != hipSuccess) return fail(PDPD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (b.p) ctx->graveyard.push_back(b.p);
+    b.p = p;
+    b.n = cap;
+    return PDPD_OK;
+}
+
+// After the stream is idle: free the buffers replaced while it ran.
+void free_graveyard(pdpd_ctx *ctx)
+{
+    for (void *q : ctx->graveyard) (void)hipFree(q);
+    ctx->graveyard.clear();
+}
+
+// Pinned staging for one relayout's uploads (total bytes known in advance):
+// waits only for the previous relayout's copies, which precede everything
+// enqueued since.
+int stage_begin(pdpd_ctx *ctx, size_t total)
+{
+    if (!ctx->stage_event) HIP_TRY(hipEventCreateWithFlags(&ctx->stage_event, hipEventDisableTiming));
+    if (ctx->stage_pending) HIP_TRY(hipEventSynchronize(ctx->stage_event));
+    ctx->stage_pending = false;
+    total += 4096;
+    if (ctx->stage_cap < total) {
+        if (ctx->stage) HIP_TRY(hipHostFree(ctx->stage));
+        ctx->stage = nullptr;
+        ctx->stage_cap = 0;
+        const size_t cap = std::max(total, ctx->stage_cap + ctx->stage_cap / 2);
+        HIP_TRY(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
+        ctx->stage_cap = cap;
+    }
+    ctx->stage_used = 0;
+    return PDPD_OK;
+}
+
+int stage_put(pdpd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return PDPD_OK;
+    if (ctx->stage_used + bytes > ctx->stage_cap) return fail(PDPD_ERR_STATE, "relayout staging overflow");
+    char *h = ctx->stage + ctx->stage_used;
+    memcpy(h, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->stage_used += (bytes + 255) & ~(size_t)255;
+    return PDPD_OK;
+}
+
+int stage_end(pdpd_ctx *ctx)
+{
+    HIP_TRY(hipEventRecord(ctx->stage_event, ctx->stream));
+    ctx->stage_pending = true;
+    return PDPD_OK;
+}
+
+template <class T>
+size_t staged_bytes(const std::vector<T> &v)
+{
+    return (v.size() * sizeof(T) + 255) & ~(size_t)255;
+}
+
 // Rebuild the device layout: resident worlds are moved to their new offsets,
-// pending worlds are uploaded (unsorted; the next step sorts them).
+// pending worlds are uploaded (unsorted; the next step sorts them). All of it
+// is enqueued on the context's stream behind the work already there -- no
+// wait for the device: buffers keep their capacity (a replaced one is freed
+// once the stream is idle), resident worlds and their bonded-term tables,
+// bead-slot map and flags are moved on the device, and host tables go up from
+// pinned staging. The host work is per world and per work item, plus the
+// upload of new worlds.
 int relayout(pdpd_ctx *ctx)
 {
+    const double t_start = now_s();
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    {
-        uint32_t any;
-        if (int rc = harvest_world_flags(ctx, &any)) return rc;
-    }
+    if (!ctx->graveyard.empty() && hipStreamQuery(ctx->stream) == hipSuccess) free_graveyard(ctx);
     std::vector<int> order;
     for (auto &kv : ctx->worlds) order.push_back(kv.first);
+    const size_t W = order.size();
+    if (W >= (1u << (32 - kTermWorldShift))) return fail(PDPD_ERR_INVALID, "too many worlds in one context");
 
-    uint64_t nb = 0, nc = 0;
-    std::vector<uint32_t> new_bead_off(order.size()), new_cell_off(order.size());
-    for (size_t i = 0; i < order.size(); i++) {
+    uint64_t nb = 0, nc = 0, ntl = 0, nterm = 0;
+    std::vector<uint32_t> new_bead_off(W), new_cell_off(W), new_list_base(W), new_term_base(W);
+    for (size_t i = 0; i < W; i++) {
         const WorldHost &w = ctx->worlds.at(order[i]);
         new_bead_off[i] = (uint32_t)nb;
         new_cell_off[i] = (uint32_t)nc;
+        new_list_base[i] = (uint32_t)ntl;
+        new_term_base[i] = (uint32_t)nterm;
         nb += w.n;
         nc += w.n_cells();
+        ntl += w.term_list_local.size();
+        nterm += w.bonds.size() + w.angles.size();
     }
     if (nb >= kMaxLayoutBeads) return fail(PDPD_ERR_INVALID, "resident worlds exceed 2^26 beads in one context");
     if (nc >= 0x7fffffffull) return fail(PDPD_ERR_INVALID, "resident worlds exceed 2^31 cells in one context");
+    if (nterm >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
+    std::map<int, uint32_t> old_index;
+    for (uint32_t i = 0; i < ctx->order.size(); i++) old_index[ctx->order[i]] = i;
 
-    DevBuf<float4> npos[2], nvel[2], nfrc[2];
-    for (int b = 0; b < 2; b++) {
-        HIP_TRY(dev_alloc(npos[b], nb));
-        HIP_TRY(dev_alloc(nvel[b], nb));
-        HIP_TRY(dev_alloc(nfrc[b], nb));
-    }
-    // slot_of_bead describes the buffer each step starts from: moved along with
-    // resident worlds, the identity for uploaded ones
-    DevBuf<uint32_t> nsob;
-    HIP_TRY(dev_alloc(nsob, nb));
-    for (size_t i = 0; i < order.size(); i++) {
-        WorldHost &w = ctx->worlds.at(order[i]);
-        const uint32_t off = new_bead_off[i];
-        if (w.n == 0) continue;
-        if (w.resident) {
-            const int c = ctx->cur;
-            HIP_TRY(hipMemcpyAsync(npos[0].p + off, ctx->pos[c].p + w.bead_off, w.n * sizeof(float4),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(nvel[0].p + off, ctx->vel[c].p + w.bead_off, w.n * sizeof(float4),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(nfrc[0].p + off, ctx->frc[c].p + w.bead_off, w.n * sizeof(float4),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(launch_rebase_index(nvel[0].p + off, w.n, off - w.bead_off, ctx->stream));
-            HIP_TRY(launch_rebase_slots(ctx->slot_of_bead.p, nsob.p, w.n, w.bead_off, off, ctx->stream));
-        } else {
-            HIP_TRY(launch_rebase_slots(nullptr, nsob.p, w.n, 0, off, ctx->stream));
-            for (uint32_t k = 0; k < w.n; k++) w.up_vel[k].w = u2f(off + k);
-            HIP_TRY(hipMemcpyAsync(npos[0].p + off, w.up_pos.data(), w.n * sizeof(float4), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            HIP_TRY(hipMemcpyAsync(nvel[0].p + off, w.up_vel.data(), w.n * sizeof(float4), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            HIP_TRY(hipMemcpyAsync(nfrc[0].p + off, w.up_frc.data(), w.n * sizeof(float4), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        }
-    }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int b = 0; b < 2; b++) {
-        ctx->pos[b].release(); ctx->vel[b].release(); ctx->frc[b].release();
-        ctx->pos[b] = npos[b]; ctx->vel[b] = nvel[b]; ctx->frc[b] = nfrc[b];
-    }
-    ctx->slot_of_bead.release();
-    ctx->slot_of_bead = nsob;
-    ctx->cur = 0;
-    ctx->cpar = 0;
-    std::vector<const WorldHost *> ws;
-    for (size_t i = 0; i < order.size(); i++) {
-        WorldHost &w = ctx->worlds.at(order[i]);
-        w.bead_off = new_bead_off[i];
-        w.cell_off = new_cell_off[i];
-        if (!w.resident) {
-            std::vector<float4>().swap(w.up_pos);
-            std::vector<float4>().swap(w.up_vel);
-            std::vector<float4>().swap(w.up_frc);
-            w.resident = true;
-        }
-        ws.push_back(&w);
-    }
-    ctx->order = order;
-    ctx->n_beads = (uint32_t)nb;
-    ctx->n_cells = (uint32_t)nc;
-    ctx->n_worlds = (uint32_t)order.size();
-
-    HIP_TRY(dev_alloc(ctx->key, nb));
-    HIP_TRY(dev_alloc(ctx->perm, nb));
-    HIP_TRY(dev_alloc(ctx->sorted_idx, nb));
-    HIP_TRY(dev_alloc(ctx->cell_count, nc));
-    HIP_TRY(dev_alloc(ctx->cell_start, nc + 1));
-    HIP_TRY(dev_alloc(ctx->scan_tiles, scan_tiles((uint32_t)nc)));
-    HIP_TRY(dev_alloc(ctx->tile_sum, scan_tiles((uint32_t)nc)));
-    if (!ctx->flags.p) {
-        HIP_TRY(dev_alloc(ctx->flags, 1));
-        HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
-    }
-    if (!ctx->scan_ticket.p) HIP_TRY(dev_alloc(ctx->scan_ticket, 1));
-    HIP_TRY(dev_alloc(ctx->wflags, std::max<size_t>(1, order.size())));
-    HIP_TRY(launch_fill_u32(ctx->wflags.p, (uint32_t)std::max<size_t>(1, order.size()), 0, ctx->stream));
-    HIP_TRY(launch_fill_u32(ctx->cell_count.p, (uint32_t)nc, 0, ctx->stream));
-
-    // world table; per-cell strengths of the worlds that have them, concatenated
-    std::vector<WorldDev> wd(order.size());
+    // ---- host tables of the new layout (small: per world and per work item)
+    std::vector<WorldDev> wd(W);
     std::vector<float2> cpar;
-    for (size_t i = 0; i < order.size(); i++) {
-        const WorldHost &w = *ws[i];
+    std::vector<const WorldHost *> ws;
+    std::vector<WorldHost> shadow(W);      // offsets of the new layout for the work-list builders
+    for (size_t i = 0; i < W; i++) {
+        const WorldHost &w = ctx->worlds.at(order[i]);
         WorldDev &d = wd[i];
         memset(&d, 0, sizeof(d));
         for (int k = 0; k < 3; k++) { d.L[k] = w.L[k]; d.Lf[k] = (float)w.L[k]; }
         d.dt = (float)w.dt;
         d.hdt = 0.5f * d.dt;
         d.rng_scale = (float)std::sqrt(24.0 / (double)d.dt);
-        d.bead_off = w.bead_off; d.n_beads = w.n;
-        d.cell_off = w.cell_off; d.n_cells = w.n_cells();
+        d.bead_off = new_bead_off[i]; d.n_beads = w.n;
+        d.cell_off = new_cell_off[i]; d.n_cells = w.n_cells();
         d.seed_lo = (uint32_t)w.seed; d.seed_hi = (uint32_t)(w.seed >> 32);
         d.n_types = w.n_types;
         d.stationary = w.stationary;
@@ -519,67 +544,148 @@ int relayout(pdpd_ctx *ctx)
             d.cell_par_off = (uint32_t)cpar.size();
             cpar.insert(cpar.end(), w.cell_par.begin(), w.cell_par.end());
         }
+        WorldHost &sh = shadow[i];
+        for (int k = 0; k < 3; k++) sh.L[k] = w.L[k];
+        sh.n = w.n;
+        sh.bead_off = new_bead_off[i];
+        sh.cell_off = new_cell_off[i];
+        ws.push_back(&sh);
     }
-    HIP_TRY(dev_upload(ctx->wdev, wd));
-    HIP_TRY(dev_upload(ctx->cell_par, cpar));
-    ctx->spatial = !cpar.empty();
-    HIP_TRY(dev_alloc(ctx->ctr, 2 * std::max<size_t>(1, order.size())));
-    ctx->ctr_host.assign(2 * std::max<size_t>(1, order.size()), WorldCtr{0, 0});
-
     std::vector<BeadBlock> bb, fb;
     build_blocks(ws, ctx->band_rows, bb, fb);
-    ctx->n_bblocks = (uint32_t)bb.size();
-    ctx->n_fblocks = (uint32_t)fb.size();
-    HIP_TRY(dev_upload(ctx->bblocks, bb));
-    HIP_TRY(dev_upload(ctx->fblocks, fb));
-    if (int rc = upload_bricks(ctx, ws)) return rc;
-    {   // first bead block of each world (the blocks are world-contiguous)
-        std::vector<uint32_t> first(order.size() + 1, (uint32_t)bb.size());
-        for (size_t b = bb.size(); b-- > 0;) first[(size_t)bb[b].world] = (uint32_t)b;
-        for (size_t i = order.size(); i-- > 0;)
-            if (first[i] > first[i + 1]) first[i] = first[i + 1];     // a world without beads has no blocks
-        HIP_TRY(dev_upload(ctx->obs_first, first));
-    }
-
-    // bonded terms with global bead indices; per bead (global index) its terms
-    // in the order the per-polymer loop applies them: its bonds, then its
-    // angles (each world's lists are built once, when it is added)
-    std::vector<TermRec> trec;
-    std::vector<uint32_t> tstart(nb + 1, 0), terms;
-    if (ws.size() >= (1u << (32 - kTermWorldShift))) return fail(PDPD_ERR_INVALID, "too many worlds in one context");
-    for (size_t i = 0; i < ws.size(); i++) {
-        WorldHost &wm = ctx->worlds.at(order[i]);
-        wm.term_base = (uint32_t)trec.size();
-        wm.t_at_layout = wm.t;
-        const WorldHost &w = *ws[i];
-        const uint32_t base = (uint32_t)trec.size();
-        const uint32_t tstart0 = (uint32_t)terms.size();
-        for (uint32_t b = 0; b < w.n; b++) tstart[w.bead_off + b] = tstart0 + (w.term_start_local.empty() ? 0 : w.term_start_local[b]);
-        if (w.bonds.empty() && w.angles.empty()) continue;
-        if (base + w.bonds.size() + w.angles.size() >= (1u << kRoleShift)) return fail(PDPD_ERR_INVALID, "too many bond terms");
-        for (const auto *list : {&w.bonds, &w.angles})
-            for (const TermRec &r0 : *list) {
-                TermRec r = r0;
-                r.a += w.bead_off; r.b += w.bead_off; r.c += w.bead_off;
-                r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
-                trec.push_back(r);
-            }
-        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
-        for (uint32_t e : w.term_list_local) terms.push_back((e & ~kIndexMask) | ((e & kIndexMask) + base));
-    }
-    tstart[nb] = (uint32_t)terms.size();
-    HIP_TRY(dev_upload(ctx->terms, trec));
-    HIP_TRY(dev_upload(ctx->term_start, tstart));
-    HIP_TRY(dev_upload(ctx->bead_terms, terms));
-    ctx->n_terms = (uint32_t)trec.size();
-    HIP_TRY(dev_alloc(ctx->term_frc, std::max<size_t>(1, trec.size()) * kTermFrcStride));
-
+    std::vector<BrickRec> bricks;
+    build_bricks(ws, ctx->brick_shape, bricks);
+    std::vector<uint32_t> first(W + 1, (uint32_t)bb.size());   // first bead block of each world
+    for (size_t b = bb.size(); b-- > 0;) first[(size_t)bb[b].world] = (uint32_t)b;
+    for (size_t i = W; i-- > 0;)
+        if (first[i] > first[i + 1]) first[i] = first[i + 1];
     std::vector<StationaryBead> sb;
-    for (size_t i = 0; i < ws.size(); i++)
-        for (uint32_t b : ws[i]->stationary_beads) sb.push_back({ws[i]->bead_off + b, (int32_t)i});
-    ctx->n_stationary = (uint32_t)sb.size();
-    HIP_TRY(dev_upload(ctx->stationary, sb));
+    for (size_t i = 0; i < W; i++)
+        for (uint32_t b : ctx->worlds.at(order[i]).stationary_beads) sb.push_back({new_bead_off[i] + b, (int32_t)i});
+    std::vector<uint32_t> remap(std::max<size_t>(1, W), ~0u);
+    for (size_t i = 0; i < W; i++) {
+        const WorldHost &w = ctx->worlds.at(order[i]);
+        auto it = old_index.find(order[i]);
+        if (w.resident && it != old_index.end()) remap[i] = it->second;
+    }
+    const std::vector<uint32_t> tstart_end{(uint32_t)ntl};
 
+    // ---- device buffers of the new layout (capacity kept across relayouts)
+    const int src = ctx->cur, dst = ctx->cur ^ 1;
+    int rc;
+    if ((rc = reserve(ctx, ctx->pos[dst], nb)) || (rc = reserve(ctx, ctx->vel[dst], nb)) ||
+        (rc = reserve(ctx, ctx->frc[dst], nb)) || (rc = reserve(ctx, ctx->slot_of_bead_alt, nb)) ||
+        (rc = reserve(ctx, ctx->term_start_alt, nb + 1)) || (rc = reserve(ctx, ctx->bead_terms_alt, ntl)) ||
+        (rc = reserve(ctx, ctx->terms_alt, nterm)) || (rc = reserve(ctx, ctx->wflags_alt, W)) ||
+        (rc = reserve(ctx, ctx->remap, W)))
+        return rc;
+
+    // ---- uploads: pending worlds' state and bonded-term tables, host tables
+    size_t need = staged_bytes(wd) + staged_bytes(cpar) + staged_bytes(bb) + staged_bytes(fb) + staged_bytes(bricks) +
+                  staged_bytes(first) + staged_bytes(sb) + staged_bytes(remap) + staged_bytes(tstart_end) +
+                  4 * sizeof(NextBuffers) + 1024;
+    for (size_t i = 0; i < W; i++) {
+        const WorldHost &w = ctx->worlds.at(order[i]);
+        if (w.resident && remap[i] != ~0u) continue;
+        need += 3 * staged_bytes(w.up_pos) + 2 * ((w.n + 1) * 4 + 255) + staged_bytes(w.term_list_local) +
+                (w.bonds.size() + w.angles.size()) * sizeof(TermRec) + 256;
+    }
+    if ((rc = stage_begin(ctx, need))) return rc;
+
+    // resident worlds: state, slot map, bonded-term tables moved on the device;
+    // pending worlds: uploaded with their tables in place
+    std::vector<TermRec> trec;
+    std::vector<uint32_t> tst, tl, sob;
+    for (size_t i = 0; i < W; i++) {
+        WorldHost &w = ctx->worlds.at(order[i]);
+        const uint32_t off = new_bead_off[i];
+        const uint32_t n_terms_w = (uint32_t)(w.bonds.size() + w.angles.size());
+        if (w.resident && remap[i] != ~0u) {
+            HIP_TRY(launch_move_world(ctx->pos[src].p, ctx->vel[src].p, ctx->frc[src].p, ctx->pos[dst].p,
+                                      ctx->vel[dst].p, ctx->frc[dst].p, w.n, w.bead_off, off, ctx->stream));
+            HIP_TRY(launch_rebase_slots(ctx->slot_of_bead.p, ctx->slot_of_bead_alt.p, w.n, w.bead_off, off,
+                                        ctx->stream));
+            HIP_TRY(launch_move_terms(ctx->term_start.p + w.bead_off, ctx->term_start_alt.p + off, w.n,
+                                      new_list_base[i] - w.list_base, ctx->bead_terms.p + w.list_base,
+                                      ctx->bead_terms_alt.p + new_list_base[i], (uint32_t)w.term_list_local.size(),
+                                      new_term_base[i] - w.term_base, ctx->terms.p + w.term_base,
+                                      ctx->terms_alt.p + new_term_base[i], n_terms_w, off - w.bead_off, (uint32_t)i,
+                                      ctx->stream));
+        } else {
+            if (w.n) {
+                for (uint32_t k = 0; k < w.n; k++) w.up_vel[k].w = u2f(off + k);
+                if ((rc = stage_put(ctx, ctx->pos[dst].p + off, w.up_pos.data(), w.n * sizeof(float4))) ||
+                    (rc = stage_put(ctx, ctx->vel[dst].p + off, w.up_vel.data(), w.n * sizeof(float4))) ||
+                    (rc = stage_put(ctx, ctx->frc[dst].p + off, w.up_frc.data(), w.n * sizeof(float4))))
+                    return rc;
+                HIP_TRY(launch_rebase_slots(nullptr, ctx->slot_of_bead_alt.p, w.n, 0, off, ctx->stream));
+            }
+            tst.resize(w.n);
+            for (uint32_t b = 0; b < w.n; b++)
+                tst[b] = new_list_base[i] + (w.term_start_local.empty() ? 0u : w.term_start_local[b]);
+            if ((rc = stage_put(ctx, ctx->term_start_alt.p + off, tst.data(), tst.size() * 4))) return rc;
+            constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
+            tl.resize(w.term_list_local.size());
+            for (size_t k = 0; k < tl.size(); k++) {
+                const uint32_t e = w.term_list_local[k];
+                tl[k] = (e & ~kIndexMask) | ((e & kIndexMask) + new_term_base[i]);
+            }
+            if ((rc = stage_put(ctx, ctx->bead_terms_alt.p + new_list_base[i], tl.data(), tl.size() * 4))) return rc;
+            trec.clear();
+            for (const auto *list : {&w.bonds, &w.angles})
+                for (const TermRec &r0 : *list) {
+                    TermRec r = r0;
+                    r.a += off; r.b += off; r.c += off;
+                    r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
+                    trec.push_back(r);
+                }
+            if ((rc = stage_put(ctx, ctx->terms_alt.p + new_term_base[i], trec.data(), trec.size() * sizeof(TermRec))))
+                return rc;
+        }
+    }
+    if ((rc = stage_put(ctx, ctx->term_start_alt.p + nb, tstart_end.data(), 4))) return rc;
+    // per-world flags follow their worlds (removed worlds' are dropped)
+    if ((rc = stage_put(ctx, ctx->remap.p, remap.data(), remap.size() * 4))) return rc;
+    if (ctx->wflags.p && !ctx->order.empty())
+        HIP_TRY(launch_remap_u32(ctx->wflags_alt.p, ctx->wflags.p, ctx->remap.p, (uint32_t)W, ctx->stream));
+    else
+        HIP_TRY(launch_fill_u32(ctx->wflags_alt.p, (uint32_t)std::max<size_t>(1, W), 0, ctx->stream));
+    std::swap(ctx->slot_of_bead, ctx->slot_of_bead_alt);
+    std::swap(ctx->term_start, ctx->term_start_alt);
+    std::swap(ctx->bead_terms, ctx->bead_terms_alt);
+    std::swap(ctx->terms, ctx->terms_alt);
+    std::swap(ctx->wflags, ctx->wflags_alt);
+    if ((rc = reserve(ctx, ctx->pos[src], nb)) || (rc = reserve(ctx, ctx->vel[src], nb)) ||
+        (rc = reserve(ctx, ctx->frc[src], nb)))
+        return rc;
+    ctx->cur = dst;
+    ctx->cpar = 0;
+
+    // ---- remaining per-bead, per-cell and per-world buffers
+    if ((rc = reserve(ctx, ctx->key, nb)) || (rc = reserve(ctx, ctx->perm, nb)) ||
+        (rc = reserve(ctx, ctx->sorted_idx, nb)) || (rc = reserve(ctx, ctx->cell_count, nc)) ||
+        (rc = reserve(ctx, ctx->cell_start, nc + 1)) || (rc = reserve(ctx, ctx->scan_tiles, scan_tiles((uint32_t)nc))) ||
+        (rc = reserve(ctx, ctx->tile_sum, scan_tiles((uint32_t)nc))) || (rc = reserve(ctx, ctx->wdev, W)) ||
+        (rc = reserve(ctx, ctx->cell_par, cpar.size())) || (rc = reserve(ctx, ctx->ctr, 2 * std::max<size_t>(1, W))) ||
+        (rc = reserve(ctx, ctx->bblocks, bb.size())) || (rc = reserve(ctx, ctx->fblocks, fb.size())) ||
+        (rc = reserve(ctx, ctx->bricks, bricks.size())) || (rc = reserve(ctx, ctx->obs_first, first.size())) ||
+        (rc = reserve(ctx, ctx->term_frc, std::max<size_t>(1, nterm) * kTermFrcStride)) ||
+        (rc = reserve(ctx, ctx->stationary, sb.size())) || (rc = reserve(ctx, ctx->next, 4)))
+        return rc;
+    if (!ctx->flags.p) {
+        HIP_TRY(dev_alloc(ctx->flags, 1));
+        HIP_TRY(launch_fill_u32(ctx->flags.p, 1, 0, ctx->stream));
+    }
+    if (!ctx->scan_ticket.p) HIP_TRY(dev_alloc(ctx->scan_ticket, 1));
+    HIP_TRY(launch_fill_u32(ctx->cell_count.p, (uint32_t)nc, 0, ctx->stream));
+    if ((rc = stage_put(ctx, ctx->wdev.p, wd.data(), wd.size() * sizeof(WorldDev))) ||
+        (rc = stage_put(ctx, ctx->cell_par.p, cpar.data(), cpar.size() * sizeof(float2))) ||
+        (rc = stage_put(ctx, ctx->bblocks.p, bb.data(), bb.size() * sizeof(BeadBlock))) ||
+        (rc = stage_put(ctx, ctx->fblocks.p, fb.data(), fb.size() * sizeof(BeadBlock))) ||
+        (rc = stage_put(ctx, ctx->bricks.p, bricks.data(), bricks.size() * sizeof(BrickRec))) ||
+        (rc = stage_put(ctx, ctx->obs_first.p, first.data(), first.size() * 4)) ||
+        (rc = stage_put(ctx, ctx->stationary.p, sb.data(), sb.size() * sizeof(StationaryBead))))
+        return rc;
     std::vector<NextBuffers> nbuf(4);
     for (int c = 0; c < 2; c++)
         for (int cp = 0; cp < 2; cp++) {
@@ -587,14 +693,48 @@ int relayout(pdpd_ctx *ctx)
             b.pos = ctx->pos[c].p; b.vel = ctx->vel[c].p; b.frc = ctx->frc[c].p;
             b.key = ctx->key.p; b.cell_count = ctx->cell_count.p; b.flags = ctx->flags.p; b.wflags = ctx->wflags.p;
             b.scan_tiles = ctx->scan_tiles.p; b.scan_ticket = ctx->scan_ticket.p;
-            b.ctr = ctx->ctr.p + (cp ^ 1) * ctx->n_worlds;
+            b.ctr = ctx->ctr.p + (cp ^ 1) * std::max<size_t>(1, W);
             b.n_tiles = scan_tiles((uint32_t)nc);
         }
-    HIP_TRY(dev_upload(ctx->next, nbuf));
+    if ((rc = stage_put(ctx, ctx->next.p, nbuf.data(), nbuf.size() * sizeof(NextBuffers)))) return rc;
+    if ((rc = stage_end(ctx))) return rc;
 
+    // ---- host bookkeeping
+    for (size_t i = 0; i < W; i++) {
+        WorldHost &w = ctx->worlds.at(order[i]);
+        w.bead_off = new_bead_off[i];
+        w.cell_off = new_cell_off[i];
+        w.list_base = new_list_base[i];
+        w.term_base = new_term_base[i];
+        w.t_at_layout = w.t;
+        if (!w.resident) {
+            std::vector<float4>().swap(w.up_pos);
+            std::vector<float4>().swap(w.up_vel);
+            std::vector<float4>().swap(w.up_frc);
+            w.resident = true;
+        }
+    }
+    ctx->order = order;
+    ctx->n_beads = (uint32_t)nb;
+    ctx->n_cells = (uint32_t)nc;
+    ctx->n_worlds = (uint32_t)W;
+    ctx->n_bblocks = (uint32_t)bb.size();
+    ctx->n_fblocks = (uint32_t)fb.size();
+    ctx->n_bricks = (uint32_t)bricks.size();
+    ctx->n_stationary = (uint32_t)sb.size();
+    ctx->n_terms = (uint32_t)nterm;
+    ctx->spatial = !cpar.empty();
+    ctx->ctr_host.assign(2 * std::max<size_t>(1, W), WorldCtr{0, 0});
     ctx->stats.n_worlds = ctx->n_worlds;
     ctx->stats.n_beads = ctx->n_beads;
     ctx->stats.n_force_blocks = ctx->n_fblocks;
+    const double took = now_s() - t_start;
+    ctx->stats.relayouts++;
+    ctx->stats.relayout_s += took;
+    if (took > ctx->stats.relayout_max_s) {
+        ctx->stats.relayout_max_s = took;
+        ctx->stats.relayout_max_beads = nb;
+    }
     ctx->layout_version++;
     ctx->steps_on_layout = 0;
     ctx->graph_version = ~0ull;
@@ -1022,6 +1162,7 @@ int pdpd_sync(pdpd_ctx *ctx)
     double t0 = now_s();
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    free_graveyard(ctx);
     ctx->stats.run_s += now_s() - t0;
     if (ctx->flags.p) return check_flags(ctx);
     return PDPD_OK;

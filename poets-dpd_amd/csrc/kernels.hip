@@ -926,6 +926,55 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+// A resident world moved from bead offset old_off (buffers s*) to new_off
+// (buffers d*): positions, velocities (bead index rebased) and forces.
+__global__ void k_move_world(const float4 *__restrict__ sp, const float4 *__restrict__ sv,
+                             const float4 *__restrict__ sf, float4 *__restrict__ dp, float4 *__restrict__ dv,
+                             float4 *__restrict__ df, uint32_t n, uint32_t old_off, uint32_t new_off)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    dp[new_off + b] = sp[old_off + b];
+    float4 v = sv[old_off + b];
+    v.w = __uint_as_float(__float_as_uint(v.w) - old_off + new_off);
+    dv[new_off + b] = v;
+    df[new_off + b] = sf[old_off + b];
+}
+
+// One resident world's bonded-term tables moved to a new layout (each pointer
+// at the world's first entry): its beads' term-list starts shifted by the
+// change d_list of the world's list base, its list entries' term indices by
+// the change d_term of its first term index, its term records' bead indices
+// by the change d_bead of its bead offset, with the world's new index. The
+// differences are taken modulo 2^32.
+__global__ void k_move_terms(const uint32_t *__restrict__ s_start, uint32_t *__restrict__ d_start, uint32_t n,
+                             uint32_t d_list, const uint32_t *__restrict__ s_list, uint32_t *__restrict__ d_lst,
+                             uint32_t n_list, uint32_t d_term, const TermRec *__restrict__ s_terms,
+                             TermRec *__restrict__ d_terms, uint32_t n_terms, uint32_t d_bead, uint32_t world)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d_start[i] = s_start[i] + d_list;
+    if (i < n_list) {
+        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
+        const uint32_t e = s_list[i];
+        d_lst[i] = (e & ~kIndexMask) | (((e & kIndexMask) + d_term) & kIndexMask);
+    }
+    if (i < n_terms) {
+        TermRec r = s_terms[i];
+        r.a += d_bead; r.b += d_bead; r.c += d_bead;
+        r.kind = (r.kind & 0xffu) | (world << kTermWorldShift);
+        d_terms[i] = r;
+    }
+}
+
+// Per-world flags carried to a new layout: nw[i] = old[map[i]], 0 for map ~0.
+__global__ void k_remap_u32(uint32_t *__restrict__ nw, const uint32_t *__restrict__ old,
+                            const uint32_t *__restrict__ map, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) nw[i] = map[i] == ~0u ? 0u : old[map[i]];
+}
+
 __global__ void k_rebase_index(float4 *__restrict__ vel, uint32_t n, uint32_t delta)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1026,6 +1075,33 @@ hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, ui
 {
     if (n == 0) return hipSuccess;
     k_rebase_slots<<<ceil_div(n, 256), 256, 0, st>>>(old, nw, n, old_off, new_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_world(const float4 *sp, const float4 *sv, const float4 *sf, float4 *dp, float4 *dv, float4 *df,
+                             uint32_t n, uint32_t old_off, uint32_t new_off, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    k_move_world<<<ceil_div(n, 256), 256, 0, st>>>(sp, sv, sf, dp, dv, df, n, old_off, new_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_terms(const uint32_t *s_start, uint32_t *d_start, uint32_t n, uint32_t d_list,
+                             const uint32_t *s_list, uint32_t *d_lst, uint32_t n_list, uint32_t d_term,
+                             const TermRec *s_terms, TermRec *d_terms, uint32_t n_terms, uint32_t d_bead,
+                             uint32_t world, hipStream_t st)
+{
+    const uint32_t m = std::max(n, std::max(n_list, n_terms));
+    if (m == 0) return hipSuccess;
+    k_move_terms<<<ceil_div(m, 256), 256, 0, st>>>(s_start, d_start, n, d_list, s_list, d_lst, n_list, d_term,
+                                                    s_terms, d_terms, n_terms, d_bead, world);
+    return hipGetLastError();
+}
+
+hipError_t launch_remap_u32(uint32_t *nw, const uint32_t *old, const uint32_t *map, uint32_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    k_remap_u32<<<ceil_div(n, 256), 256, 0, st>>>(nw, old, map, n);
     return hipGetLastError();
 }
 

@@ -309,6 +309,11 @@ int pdpd_server_run(pdpd_server *s)
             continue;
         }
         s->stats.max_resident = std::max<uint64_t>(s->stats.max_resident, s->resident.size());
+        {
+            uint64_t rb = 0;
+            for (uint64_t pb : s->part_beads) rb += pb;
+            s->stats.max_resident_beads = std::max(s->stats.max_resident_beads, rb);
+        }
         uint32_t k = 0;
         for (Request *r : s->resident) k = std::max(k, std::min(K, r->steps - r->done - r->inflight));
         std::vector<std::pair<Request *, uint32_t>> next;
@@ -349,6 +354,19 @@ int pdpd_server_run(pdpd_server *s)
         if (rc) return fail_rc(rc);
     }
     s->stats.makespan_s = rel();
+    {
+        double sum = 0;
+        s->stats.relayouts = 0;
+        s->stats.relayout_max_s = 0;
+        for (pdpd_ctx *c : s->ctxs) {
+            pdpd_stats cs;
+            if (pdpd_get_stats(c, &cs) != PDPD_OK) continue;
+            s->stats.relayouts += cs.relayouts;
+            sum += cs.relayout_s;
+            s->stats.relayout_max_s = std::max(s->stats.relayout_max_s, cs.relayout_max_s);
+        }
+        s->stats.relayout_mean_s = s->stats.relayouts ? sum / (double)s->stats.relayouts : 0.0;
+    }
     s->stats.gpu_busy_s += s->stats.makespan_s - idle;
     s->stats.requests = s->reqs.size();
     std::vector<double> lat;

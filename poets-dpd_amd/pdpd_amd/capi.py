@@ -23,7 +23,7 @@ class PdpdError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 4          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
+ABI_VERSION = 5          # PDPD_ABI_VERSION of the pdpd_world_desc layout below
 
 
 class WorldDesc(C.Structure):
@@ -57,7 +57,8 @@ class Stats(C.Structure):
                 ("n_beads", C.c_uint64), ("n_force_blocks", C.c_uint32),
                 ("k_integrate_ms", C.c_double), ("k_scan_ms", C.c_double), ("k_scatter_ms", C.c_double),
                 ("k_reorder_ms", C.c_double), ("k_force_ms", C.c_double), ("k_stationary_ms", C.c_double),
-                ("profiled_steps", C.c_uint64)]
+                ("profiled_steps", C.c_uint64), ("relayouts", C.c_uint64), ("relayout_s", C.c_double),
+                ("relayout_max_s", C.c_double), ("relayout_max_beads", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,7 +81,8 @@ class ServerStats(C.Structure):
                 ("max_resident", C.c_uint64), ("makespan_s", C.c_double), ("gpu_busy_s", C.c_double),
                 ("bead_steps", C.c_double), ("bead_steps_per_s", C.c_double), ("p50_latency_s", C.c_double),
                 ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("max_latency_s", C.c_double),
-                ("failed", C.c_uint64)]
+                ("failed", C.c_uint64), ("relayouts", C.c_uint64), ("max_resident_beads", C.c_uint64),
+                ("relayout_mean_s", C.c_double), ("relayout_max_s", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

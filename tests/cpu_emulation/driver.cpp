@@ -259,6 +259,51 @@ int main(int argc, char **argv)
         }
         fprintf(stderr, "  ok %zu worlds over %zu contexts\n", batch.size(), batch.parts());
     }
+    {   // worlds retired and admitted between runs: resident worlds (bonded ones
+        // too) move to new offsets on the device and go on as if run alone
+        fprintf(stderr, "case relayout\n");
+        W a, b, c, e;
+        make(a, 5, 4, 6, 0.2, false, 0.0, 31);
+        make(b, 6, 5, 5, 0.2, true, 0.5, 32);
+        make(c, 4, 4, 5, 0.0, false, 0.0, 33);
+        make(e, 4, 5, 4, 0.2, false, 0.0, 34);
+        pdpd_ctx *ctx;
+        CHECK(pdpd_ctx_create(0, &ctx));
+        int sa, sb, sc, se;
+        CHECK(pdpd_world_add(ctx, &a.d, &sa));
+        CHECK(pdpd_world_add(ctx, &b.d, &sb));
+        CHECK(pdpd_world_add(ctx, &c.d, &sc));
+        CHECK(pdpd_run(ctx, 3));
+        CHECK(pdpd_world_remove(ctx, sa));
+        CHECK(pdpd_world_add(ctx, &e.d, &se));
+        CHECK(pdpd_run(ctx, 2));
+        CHECK(pdpd_world_remove(ctx, se));
+        CHECK(pdpd_run(ctx, 1));
+        CHECK(pdpd_sync(ctx));
+        pdpd_stats stt;
+        CHECK(pdpd_get_stats(ctx, &stt));
+        if (stt.relayouts != 3) { fprintf(stderr, "relayouts %llu, expected 3\n", (unsigned long long)stt.relayouts); return 1; }
+        const struct { W *w; int slot; } kept[] = {{&b, sb}, {&c, sc}};
+        for (const auto &k : kept) {
+            const size_t n3 = 3 * k.w->d.n_beads;
+            std::vector<double> x(n3), v(n3), f(n3), x1(n3), v1(n3), f1(n3);
+            int64_t t, t1;
+            CHECK(pdpd_world_download(ctx, k.slot, x.data(), v.data(), f.data(), &t));
+            pdpd_ctx *one;
+            CHECK(pdpd_ctx_create(0, &one));
+            int s1;
+            CHECK(pdpd_world_add(one, &k.w->d, &s1));
+            CHECK(pdpd_run(one, 6));
+            CHECK(pdpd_world_download(one, s1, x1.data(), v1.data(), f1.data(), &t1));
+            CHECK(pdpd_ctx_destroy(one));
+            if (t != 6 || t1 != 6 || x != x1 || v != v1 || f != f1) {
+                fprintf(stderr, "relayout: world %d differs from its solo run\n", k.slot);
+                return 1;
+            }
+        }
+        CHECK(pdpd_ctx_destroy(ctx));
+        fprintf(stderr, "  ok %llu relayouts\n", (unsigned long long)stt.relayouts);
+    }
     fprintf(stderr, "all cases done\n");
     return 0;
 }

@@ -37,7 +37,7 @@ def test_python_binding_covers_header():
 
 
 def test_abi_version_and_limits():
-    assert capi.load().pdpd_abi_version() == capi.ABI_VERSION == 4
+    assert capi.load().pdpd_abi_version() == capi.ABI_VERSION == 5
     lim = capi.limits()
     assert lim["max_bead_types"] == 16 and lim["min_box"] == 3 and lim["max_polymer_length"] == 127
 

@@ -27,14 +27,17 @@ FAMILIES = ["TestIsolatedMovingBead", "TestMultipleMovingBeads", "TestCollidingB
 MIN_TESTS = {"scenarios": 200, "differential": 200}      # the registered suite holds 225 scenarios
 
 
-def _run(mode, family, limit_s=600.0):
+def _run(mode, family, capsys, limit_s=600.0):
     """Run one scenario family (name prefix up to '['), streaming each TAP line as
-    it arrives so a long family shows progress. stderr is merged into stdout; a
-    reader thread collects the lines and the child is killed at the deadline,
-    so a hung or chatty child fails the test instead of blocking it."""
+    it arrives so a long family shows progress (output capture is suspended while
+    the child runs, so the lines reach the terminal as they come). stderr is
+    merged into stdout; a reader thread collects the lines and the child is
+    killed at the deadline, so a hung or chatty child fails the test instead of
+    blocking it."""
     if not os.path.exists(BIN):
         pytest.skip("oracle/_ref/ref_engine_tests not built (needs the reference tree at build time)")
-    print(f"reference harness: {mode} {family}", flush=True)
+    with capsys.disabled():
+        print(f"\nreference harness: {mode} {family}", flush=True)
     p = subprocess.Popen([BIN, mode, family + "["], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     lines, other = [], []
 
@@ -45,9 +48,10 @@ def _run(mode, family, limit_s=600.0):
                 print("  " + line.rstrip()[:160], flush=True)
             else:
                 other.append(line)
-    t = threading.Thread(target=reader, daemon=True)
-    t.start()
-    t.join(limit_s)
+    with capsys.disabled():
+        t = threading.Thread(target=reader, daemon=True)
+        t.start()
+        t.join(limit_s)
     if t.is_alive():
         p.kill()
         p.wait()
@@ -63,8 +67,8 @@ _counts = {}
 
 
 @pytest.mark.parametrize("family", FAMILIES)
-def test_reference_scenarios_pass_on_hip_engine(family):
-    rc, lines, failed, err = _run("scenarios", family)
+def test_reference_scenarios_pass_on_hip_engine(family, capsys):
+    rc, lines, failed, err = _run("scenarios", family, capsys)
     _counts[("scenarios", family)] = len(lines)
     assert lines, f"no {family} scenarios ran"
     assert not failed, "\n".join(failed[:10]) + "\n" + err[-2000:]
@@ -72,8 +76,8 @@ def test_reference_scenarios_pass_on_hip_engine(family):
 
 
 @pytest.mark.parametrize("family", FAMILIES)
-def test_reference_differential_vs_naive_engine(family):
-    rc, lines, failed, err = _run("differential", family)
+def test_reference_differential_vs_naive_engine(family, capsys):
+    rc, lines, failed, err = _run("differential", family, capsys)
     _counts[("differential", family)] = len(lines)
     assert lines, f"no {family} scenarios ran"
     assert not failed, "\n".join(failed[:10]) + "\n" + err[-2000:]
