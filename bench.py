@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--serve", type=int, default=256, help="served sweep with this many requests (0 = skip)")
     ap.add_argument("--serve-budget-gib", type=float, default=0.5)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
-    ap.add_argument("--only", choices=("ensemble", "polymer32", "large128"), default=None,
+    ap.add_argument("--only", choices=("ensemble", "polymer32", "large128", "served"), default=None,
                     help="run one workload only (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
@@ -323,6 +323,10 @@ def main():
         el = timed(ctx, args.steps)
         log(f"bench: {args.only}: {world.n_beads * args.steps / el:.4e} bead-steps/s")
         ctx.close()
+        return
+
+    if args.only == "served":
+        print(json.dumps(run_served(args, capi, W)), flush=True)
         return
 
     # ---- headline: the 256-world ensemble, batched engine

@@ -102,6 +102,11 @@ struct WorldHost {
     uint32_t list_base = 0;                        // index of the world's first per-bead term-list entry
     int64_t t_at_layout = 0;                       // t when the current layout was built
     uint32_t flags = 0;                            // PDPD_FLAG_* raised by this world so far
+    // work lists in local offsets, kept between layouts (see local_force_blocks, build_bricks)
+    std::vector<BeadBlock> fb_local;
+    uint32_t fb_rows = UINT32_MAX;
+    std::vector<BrickRec> bricks_local;
+    uint32_t bricks_shape = UINT32_MAX;
     uint32_t n_cells() const { return (uint32_t)L[0] * L[1] * L[2]; }
 };
 
@@ -157,14 +162,24 @@ struct pdpd_ctx {
     DevBuf<uint32_t> slot_of_bead_alt, wflags_alt, term_start_alt, bead_terms_alt;
     DevBuf<TermRec> terms_alt;
     DevBuf<uint32_t> remap;               // relayout: new world index -> old
+    DevBuf<MoveSeg> move_segs;            // relayout: one segment per world (k_move_worlds)
     // buffers replaced while the stream may still read them, freed once it is idle
     std::vector<void *> graveyard;
-    // pinned staging of a relayout's host-to-device uploads; reused once the
-    // previous relayout's copies are done (stage_event)
-    char *stage = nullptr;
+    std::vector<void *> host_graveyard;   // pinned arenas replaced while their copies may run
+    // pinned staging of relayouts' host-to-device uploads: a ring of arenas,
+    // each reused once the copies of the relayout that last filled it are done
+    // (its event), so a relayout does not wait for the steps enqueued before it
+    static constexpr int kStageArenas = 4;
+    struct StageArena {
+        char *p = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    } arenas[kStageArenas];
+    int stage_cur = 0;                    // arena of the relayout being built
+    char *stage = nullptr;                // = arenas[stage_cur].p while staging
     size_t stage_cap = 0, stage_used = 0;
-    hipEvent_t stage_event = nullptr;
-    bool stage_pending = false;
+    uint32_t reserved_beads = 0;          // reserve_beads setting
     DevBuf<unsigned long long> scan_tiles;
     DevBuf<uint32_t> tile_sum;
     DevBuf<WorldDev> wdev;
@@ -236,10 +251,13 @@ struct pdpd_ctx {
         cell_start.release(); slot_of_bead.release(); flags.release(); scan_ticket.release(); scan_tiles.release();
         wflags.release();
         slot_of_bead_alt.release(); wflags_alt.release(); term_start_alt.release(); bead_terms_alt.release();
-        terms_alt.release(); remap.release();
+        terms_alt.release(); remap.release(); move_segs.release();
         for (void *q : graveyard) (void)hipFree(q);
-        if (stage) (void)hipHostFree(stage);
-        if (stage_event) (void)hipEventDestroy(stage_event);
+        for (void *q : host_graveyard) (void)hipHostFree(q);
+        for (StageArena &a : arenas) {
+            if (a.p) (void)hipHostFree(a.p);
+            if (a.done) (void)hipEventDestroy(a.done);
+        }
         tile_sum.release();
         wdev.release(); ctr.release(); bblocks.release(); fblocks.release(); term_start.release();
         bead_terms.release(); terms.release(); term_frc.release(); stationary.release();
@@ -307,29 +325,44 @@ namespace {
 // block's cell is estimated from its middle slot at the world's mean density
 // (slots are in cell order). The order changes which XCD evaluates a block and
 // when, never a block's beads or any sum, so forces are bit-identical.
-void build_blocks(const std::vector<const WorldHost *> &ws, uint32_t band_rows, std::vector<BeadBlock> &bb,
-                  std::vector<BeadBlock> &fb)
+// The pair-force work items of one world (kForceBeads slots each) in local
+// slot offsets, in band order when band_rows is set; kept with the world and
+// rebuilt only when the band setting changes.
+const std::vector<BeadBlock> &local_force_blocks(WorldHost &w, uint32_t band_rows)
 {
-    bb.clear();
+    if (w.fb_rows == band_rows) return w.fb_local;
+    std::vector<BeadBlock> &fb = w.fb_local;
     fb.clear();
-    for (size_t i = 0; i < ws.size(); i++) {
-        const WorldHost &w = *ws[i];
-        for (uint32_t b = 0; b < w.n; b += kBeadThreads)
-            bb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kBeadThreads), 0});
-        const size_t first = fb.size();
-        for (uint32_t b = 0; b < w.n; b += kForceBeads)
-            fb.push_back({(int32_t)i, w.bead_off + b, w.bead_off + std::min(w.n, b + kForceBeads), 0});
-        const uint64_t plane = (uint64_t)w.L[1] * (uint64_t)w.L[2];
-        if (band_rows == 0 || plane < kBandMinCells || (uint32_t)w.L[1] <= band_rows || w.n == 0) continue;
+    for (uint32_t b = 0; b < w.n; b += kForceBeads) fb.push_back({0, b, std::min(w.n, b + kForceBeads), 0});
+    const uint64_t plane = (uint64_t)w.L[1] * (uint64_t)w.L[2];
+    if (!(band_rows == 0 || plane < kBandMinCells || (uint32_t)w.L[1] <= band_rows || w.n == 0)) {
         const double cells_per_slot = (double)w.n_cells() / (double)w.n;
         auto band_plane = [&](const BeadBlock &b) {
-            const uint64_t mid = ((uint64_t)(b.begin - w.bead_off) + (uint64_t)(b.end - w.bead_off)) / 2;
+            const uint64_t mid = ((uint64_t)b.begin + (uint64_t)b.end) / 2;
             const uint64_t c = std::min<uint64_t>((uint64_t)((double)mid * cells_per_slot), w.n_cells() - 1);
             const uint64_t y = (c / (uint64_t)w.L[2]) % (uint64_t)w.L[1];
             return std::make_pair(y / band_rows, c / plane);
         };
-        std::stable_sort(fb.begin() + first, fb.end(),
+        std::stable_sort(fb.begin(), fb.end(),
                          [&](const BeadBlock &a, const BeadBlock &b) { return band_plane(a) < band_plane(b); });
+    }
+    w.fb_rows = band_rows;
+    return fb;
+}
+
+// Work lists of the worlds in `order` placed at bead offsets off[i]: bead
+// blocks (kBeadThreads slots: integrate, observables) and pair-force blocks.
+void build_blocks(pdpd_ctx *ctx, const std::vector<int> &order, const std::vector<uint32_t> &off,
+                  std::vector<BeadBlock> &bb, std::vector<BeadBlock> &fb)
+{
+    bb.clear();
+    fb.clear();
+    for (size_t i = 0; i < order.size(); i++) {
+        WorldHost &w = ctx->worlds.at(order[i]);
+        for (uint32_t b = 0; b < w.n; b += kBeadThreads)
+            bb.push_back({(int32_t)i, off[i] + b, off[i] + std::min(w.n, b + kBeadThreads), 0});
+        for (const BeadBlock &l : local_force_blocks(w, ctx->band_rows))
+            fb.push_back({(int32_t)i, off[i] + l.begin, off[i] + l.end, 0});
     }
 }
 
@@ -337,33 +370,42 @@ void build_blocks(const std::vector<const WorldHost *> &ws, uint32_t band_rows, 
 // clipped at the world's far faces, listed world by world in x, y, z order
 // (the pair-force kernel deals contiguous shares of the list to the XCDs, so
 // an XCD works through neighbouring bricks whose halos overlap in its L2).
-void build_bricks(const std::vector<const WorldHost *> &ws, uint32_t shape, std::vector<BrickRec> &out)
+// Bricks name cells by world and coordinates, so a world's list is kept with
+// it and only its world index changes between layouts.
+void build_bricks(pdpd_ctx *ctx, const std::vector<int> &order, std::vector<BrickRec> &out)
 {
     uint32_t bx, by, bz;
-    brick_shape_dims(shape, &bx, &by, &bz);
+    brick_shape_dims(ctx->brick_shape, &bx, &by, &bz);
     out.clear();
-    for (size_t i = 0; i < ws.size(); i++) {
-        const WorldHost &w = *ws[i];
+    for (size_t i = 0; i < order.size(); i++) {
+        WorldHost &w = ctx->worlds.at(order[i]);
         if (w.n == 0) continue;
-        for (int x = 0; x < w.L[0]; x += (int)bx)
-            for (int y = 0; y < w.L[1]; y += (int)by)
-                for (int z = 0; z < w.L[2]; z += (int)bz) {
-                    BrickRec r{};
-                    r.world = (int32_t)i;
-                    r.origin = (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
-                    const uint32_t dx = std::min<uint32_t>(bx, (uint32_t)(w.L[0] - x));
-                    const uint32_t dy = std::min<uint32_t>(by, (uint32_t)(w.L[1] - y));
-                    const uint32_t dz = std::min<uint32_t>(bz, (uint32_t)(w.L[2] - z));
-                    r.dims = dx | (dy << 8) | (dz << 16);
-                    out.push_back(r);
-                }
+        if (w.bricks_shape != ctx->brick_shape) {
+            w.bricks_local.clear();
+            for (int x = 0; x < w.L[0]; x += (int)bx)
+                for (int y = 0; y < w.L[1]; y += (int)by)
+                    for (int z = 0; z < w.L[2]; z += (int)bz) {
+                        BrickRec r{};
+                        r.origin = (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+                        const uint32_t dx = std::min<uint32_t>(bx, (uint32_t)(w.L[0] - x));
+                        const uint32_t dy = std::min<uint32_t>(by, (uint32_t)(w.L[1] - y));
+                        const uint32_t dz = std::min<uint32_t>(bz, (uint32_t)(w.L[2] - z));
+                        r.dims = dx | (dy << 8) | (dz << 16);
+                        w.bricks_local.push_back(r);
+                    }
+            w.bricks_shape = ctx->brick_shape;
+        }
+        for (BrickRec r : w.bricks_local) {
+            r.world = (int32_t)i;
+            out.push_back(r);
+        }
     }
 }
 
-int upload_bricks(pdpd_ctx *ctx, const std::vector<const WorldHost *> &ws)
+int upload_bricks(pdpd_ctx *ctx)
 {
     std::vector<BrickRec> br;
-    build_bricks(ws, ctx->brick_shape, br);
+    build_bricks(ctx, ctx->order, br);
     ctx->n_bricks = (uint32_t)br.size();
     HIP_TRY(dev_upload(ctx->bricks, br));
     return PDPD_OK;
@@ -427,6 +469,8 @@ void free_graveyard(pdpd_ctx *ctx)
 {
     for (void *q : ctx->graveyard) (void)hipFree(q);
     ctx->graveyard.clear();
+    for (void *q : ctx->host_graveyard) (void)hipHostFree(q);
+    ctx->host_graveyard.clear();
 }
 
 // Pinned staging for one relayout's uploads (total bytes known in advance):
@@ -434,18 +478,42 @@ void free_graveyard(pdpd_ctx *ctx)
 // enqueued since.
 int stage_begin(pdpd_ctx *ctx, size_t total)
 {
-    if (!ctx->stage_event) HIP_TRY(hipEventCreateWithFlags(&ctx->stage_event, hipEventDisableTiming));
-    if (ctx->stage_pending) HIP_TRY(hipEventSynchronize(ctx->stage_event));
-    ctx->stage_pending = false;
     total += 4096;
-    if (ctx->stage_cap < total) {
-        if (ctx->stage) HIP_TRY(hipHostFree(ctx->stage));
-        ctx->stage = nullptr;
-        ctx->stage_cap = 0;
-        const size_t cap = std::max(total, ctx->stage_cap + ctx->stage_cap / 2);
-        HIP_TRY(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
-        ctx->stage_cap = cap;
+    // the first arena, from the one after the last used, whose copies are done
+    // and which is large enough; else the first done one (grown); else wait
+    // for the next in turn
+    int pick = -1, idle = -1;
+    for (int k = 1; k <= pdpd_ctx::kStageArenas && pick < 0; k++) {
+        const int i = (ctx->stage_cur + k) % pdpd_ctx::kStageArenas;
+        pdpd_ctx::StageArena &a = ctx->arenas[i];
+        if (!a.done) HIP_TRY(hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
+        if (a.pending) {
+            const hipError_t q = hipEventQuery(a.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return fail(PDPD_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+            a.pending = false;
+        }
+        if (a.cap >= total) pick = i;
+        else if (idle < 0) idle = i;
     }
+    if (pick < 0) pick = idle;
+    if (pick < 0) {
+        pick = (ctx->stage_cur + 1) % pdpd_ctx::kStageArenas;
+        HIP_TRY(hipEventSynchronize(ctx->arenas[pick].done));
+        ctx->arenas[pick].pending = false;
+    }
+    pdpd_ctx::StageArena &a = ctx->arenas[pick];
+    if (a.cap < total) {
+        const size_t cap = std::max(total, a.cap + a.cap / 2);
+        if (a.p) ctx->host_graveyard.push_back(a.p);   // freeing pinned memory may wait for the device
+        a.p = nullptr;
+        a.cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&a.p, cap, hipHostMallocDefault));
+        a.cap = cap;
+    }
+    ctx->stage_cur = pick;
+    ctx->stage = a.p;
+    ctx->stage_cap = a.cap;
     ctx->stage_used = 0;
     return PDPD_OK;
 }
@@ -463,8 +531,55 @@ int stage_put(pdpd_ctx *ctx, void *dst, const void *src, size_t bytes)
 
 int stage_end(pdpd_ctx *ctx)
 {
-    HIP_TRY(hipEventRecord(ctx->stage_event, ctx->stream));
-    ctx->stage_pending = true;
+    pdpd_ctx::StageArena &a = ctx->arenas[ctx->stage_cur];
+    HIP_TRY(hipEventRecord(a.done, ctx->stream));
+    a.pending = true;
+    ctx->stage = nullptr;
+    ctx->stage_cap = 0;
+    return PDPD_OK;
+}
+
+// Capacity for layouts of up to n beads (n cells, n / 2 bonded terms, n
+// term-list entries, 256 worlds) and the pinned staging their tables and one
+// large upload need, so that building them allocates nothing. Only before any
+// layout exists: the buffers' contents are not kept.
+int reserve_capacity(pdpd_ctx *ctx, size_t n)
+{
+    if (n == 0) return PDPD_OK;
+    int rc;
+    for (int b = 0; b < 2; b++)
+        if ((rc = reserve(ctx, ctx->pos[b], n)) || (rc = reserve(ctx, ctx->vel[b], n)) ||
+            (rc = reserve(ctx, ctx->frc[b], n)))
+            return rc;
+    const size_t W = 256, nc = n, nt = n / 2 + 1;
+    if ((rc = reserve(ctx, ctx->slot_of_bead, n)) || (rc = reserve(ctx, ctx->slot_of_bead_alt, n)) ||
+        (rc = reserve(ctx, ctx->term_start, n + 1)) || (rc = reserve(ctx, ctx->term_start_alt, n + 1)) ||
+        (rc = reserve(ctx, ctx->bead_terms, n)) || (rc = reserve(ctx, ctx->bead_terms_alt, n)) ||
+        (rc = reserve(ctx, ctx->terms, nt)) || (rc = reserve(ctx, ctx->terms_alt, nt)) ||
+        (rc = reserve(ctx, ctx->term_frc, nt * kTermFrcStride)) || (rc = reserve(ctx, ctx->key, n)) ||
+        (rc = reserve(ctx, ctx->perm, n)) || (rc = reserve(ctx, ctx->sorted_idx, n)) ||
+        (rc = reserve(ctx, ctx->cell_count, nc)) || (rc = reserve(ctx, ctx->cell_start, nc + 1)) ||
+        (rc = reserve(ctx, ctx->scan_tiles, scan_tiles((uint32_t)nc))) ||
+        (rc = reserve(ctx, ctx->tile_sum, scan_tiles((uint32_t)nc))) || (rc = reserve(ctx, ctx->wflags, W)) ||
+        (rc = reserve(ctx, ctx->wflags_alt, W)) || (rc = reserve(ctx, ctx->remap, W)) ||
+        (rc = reserve(ctx, ctx->move_segs, W)) || (rc = reserve(ctx, ctx->wdev, W)) ||
+        (rc = reserve(ctx, ctx->ctr, 2 * W)) || (rc = reserve(ctx, ctx->bblocks, n / kBeadThreads + W)) ||
+        (rc = reserve(ctx, ctx->fblocks, n / kForceBeads + W)) || (rc = reserve(ctx, ctx->bricks, nc / 8 + W)) ||
+        (rc = reserve(ctx, ctx->obs_first, W + 1)))
+        return rc;
+    // staging, per arena: the tables above for n beads plus an upload of n / 8 beads
+    const size_t stage = n / kBeadThreads * sizeof(BeadBlock) + (n / kForceBeads) * sizeof(BeadBlock) +
+                         (nc / 8) * sizeof(BrickRec) + W * (sizeof(WorldDev) + sizeof(MoveSeg) + 16) +
+                         (n / 8) * (3 * sizeof(float4) + 32) + (1u << 20);
+    for (int k = 0; k < pdpd_ctx::kStageArenas; k++) {
+        pdpd_ctx::StageArena &a = ctx->arenas[k];
+        if (a.cap >= stage) continue;
+        if (a.p) HIP_TRY(hipHostFree(a.p));      // no layout yet: no copy is pending
+        a.p = nullptr;
+        a.cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&a.p, stage, hipHostMallocDefault));
+        a.cap = stage;
+    }
     return PDPD_OK;
 }
 
@@ -486,7 +601,8 @@ int relayout(pdpd_ctx *ctx)
 {
     const double t_start = now_s();
     HIP_TRY(hipSetDevice(ctx->device));
-    if (!ctx->graveyard.empty() && hipStreamQuery(ctx->stream) == hipSuccess) free_graveyard(ctx);
+    if ((!ctx->graveyard.empty() || !ctx->host_graveyard.empty()) && hipStreamQuery(ctx->stream) == hipSuccess)
+        free_graveyard(ctx);
     std::vector<int> order;
     for (auto &kv : ctx->worlds) order.push_back(kv.first);
     const size_t W = order.size();
@@ -514,8 +630,6 @@ int relayout(pdpd_ctx *ctx)
     // ---- host tables of the new layout (small: per world and per work item)
     std::vector<WorldDev> wd(W);
     std::vector<float2> cpar;
-    std::vector<const WorldHost *> ws;
-    std::vector<WorldHost> shadow(W);      // offsets of the new layout for the work-list builders
     for (size_t i = 0; i < W; i++) {
         const WorldHost &w = ctx->worlds.at(order[i]);
         WorldDev &d = wd[i];
@@ -544,17 +658,11 @@ int relayout(pdpd_ctx *ctx)
             d.cell_par_off = (uint32_t)cpar.size();
             cpar.insert(cpar.end(), w.cell_par.begin(), w.cell_par.end());
         }
-        WorldHost &sh = shadow[i];
-        for (int k = 0; k < 3; k++) sh.L[k] = w.L[k];
-        sh.n = w.n;
-        sh.bead_off = new_bead_off[i];
-        sh.cell_off = new_cell_off[i];
-        ws.push_back(&sh);
     }
     std::vector<BeadBlock> bb, fb;
-    build_blocks(ws, ctx->band_rows, bb, fb);
+    build_blocks(ctx, order, new_bead_off, bb, fb);
     std::vector<BrickRec> bricks;
-    build_bricks(ws, ctx->brick_shape, bricks);
+    build_bricks(ctx, order, bricks);
     std::vector<uint32_t> first(W + 1, (uint32_t)bb.size());   // first bead block of each world
     for (size_t b = bb.size(); b-- > 0;) first[(size_t)bb[b].world] = (uint32_t)b;
     for (size_t i = W; i-- > 0;)
@@ -577,13 +685,13 @@ int relayout(pdpd_ctx *ctx)
         (rc = reserve(ctx, ctx->frc[dst], nb)) || (rc = reserve(ctx, ctx->slot_of_bead_alt, nb)) ||
         (rc = reserve(ctx, ctx->term_start_alt, nb + 1)) || (rc = reserve(ctx, ctx->bead_terms_alt, ntl)) ||
         (rc = reserve(ctx, ctx->terms_alt, nterm)) || (rc = reserve(ctx, ctx->wflags_alt, W)) ||
-        (rc = reserve(ctx, ctx->remap, W)))
+        (rc = reserve(ctx, ctx->remap, W)) || (rc = reserve(ctx, ctx->move_segs, W)))
         return rc;
 
     // ---- uploads: pending worlds' state and bonded-term tables, host tables
     size_t need = staged_bytes(wd) + staged_bytes(cpar) + staged_bytes(bb) + staged_bytes(fb) + staged_bytes(bricks) +
                   staged_bytes(first) + staged_bytes(sb) + staged_bytes(remap) + staged_bytes(tstart_end) +
-                  4 * sizeof(NextBuffers) + 1024;
+                  W * sizeof(MoveSeg) + 4 * sizeof(NextBuffers) + 2048;
     for (size_t i = 0; i < W; i++) {
         const WorldHost &w = ctx->worlds.at(order[i]);
         if (w.resident && remap[i] != ~0u) continue;
@@ -592,56 +700,71 @@ int relayout(pdpd_ctx *ctx)
     }
     if ((rc = stage_begin(ctx, need))) return rc;
 
-    // resident worlds: state, slot map, bonded-term tables moved on the device;
-    // pending worlds: uploaded with their tables in place
+    // resident worlds: state, slot map, bonded-term tables moved on the device
+    // (one launch for all of them); pending worlds: uploaded with their tables
+    // in place
     std::vector<TermRec> trec;
-    std::vector<uint32_t> tst, tl, sob;
+    std::vector<uint32_t> tst, tl;
+    std::vector<MoveSeg> segs;
+    uint32_t move_blocks = 0;
     for (size_t i = 0; i < W; i++) {
         WorldHost &w = ctx->worlds.at(order[i]);
         const uint32_t off = new_bead_off[i];
         const uint32_t n_terms_w = (uint32_t)(w.bonds.size() + w.angles.size());
-        if (w.resident && remap[i] != ~0u) {
-            HIP_TRY(launch_move_world(ctx->pos[src].p, ctx->vel[src].p, ctx->frc[src].p, ctx->pos[dst].p,
-                                      ctx->vel[dst].p, ctx->frc[dst].p, w.n, w.bead_off, off, ctx->stream));
-            HIP_TRY(launch_rebase_slots(ctx->slot_of_bead.p, ctx->slot_of_bead_alt.p, w.n, w.bead_off, off,
-                                        ctx->stream));
-            HIP_TRY(launch_move_terms(ctx->term_start.p + w.bead_off, ctx->term_start_alt.p + off, w.n,
-                                      new_list_base[i] - w.list_base, ctx->bead_terms.p + w.list_base,
-                                      ctx->bead_terms_alt.p + new_list_base[i], (uint32_t)w.term_list_local.size(),
-                                      new_term_base[i] - w.term_base, ctx->terms.p + w.term_base,
-                                      ctx->terms_alt.p + new_term_base[i], n_terms_w, off - w.bead_off, (uint32_t)i,
-                                      ctx->stream));
-        } else {
-            if (w.n) {
-                for (uint32_t k = 0; k < w.n; k++) w.up_vel[k].w = u2f(off + k);
-                if ((rc = stage_put(ctx, ctx->pos[dst].p + off, w.up_pos.data(), w.n * sizeof(float4))) ||
-                    (rc = stage_put(ctx, ctx->vel[dst].p + off, w.up_vel.data(), w.n * sizeof(float4))) ||
-                    (rc = stage_put(ctx, ctx->frc[dst].p + off, w.up_frc.data(), w.n * sizeof(float4))))
-                    return rc;
-                HIP_TRY(launch_rebase_slots(nullptr, ctx->slot_of_bead_alt.p, w.n, 0, off, ctx->stream));
-            }
-            tst.resize(w.n);
-            for (uint32_t b = 0; b < w.n; b++)
-                tst[b] = new_list_base[i] + (w.term_start_local.empty() ? 0u : w.term_start_local[b]);
-            if ((rc = stage_put(ctx, ctx->term_start_alt.p + off, tst.data(), tst.size() * 4))) return rc;
-            constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
-            tl.resize(w.term_list_local.size());
-            for (size_t k = 0; k < tl.size(); k++) {
-                const uint32_t e = w.term_list_local[k];
-                tl[k] = (e & ~kIndexMask) | ((e & kIndexMask) + new_term_base[i]);
-            }
-            if ((rc = stage_put(ctx, ctx->bead_terms_alt.p + new_list_base[i], tl.data(), tl.size() * 4))) return rc;
-            trec.clear();
-            for (const auto *list : {&w.bonds, &w.angles})
-                for (const TermRec &r0 : *list) {
-                    TermRec r = r0;
-                    r.a += off; r.b += off; r.c += off;
-                    r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
-                    trec.push_back(r);
-                }
-            if ((rc = stage_put(ctx, ctx->terms_alt.p + new_term_base[i], trec.data(), trec.size() * sizeof(TermRec))))
+        const uint32_t n_list_w = (uint32_t)w.term_list_local.size();
+        const bool moved = w.resident && remap[i] != ~0u;
+        MoveSeg g{};
+        g.first_block = move_blocks;
+        g.world = (uint32_t)i;
+        g.fresh = moved ? 0u : 1u;
+        g.n = w.n;
+        g.src_off = w.bead_off; g.dst_off = off;
+        g.n_list = moved ? n_list_w : 0u; g.src_list = w.list_base; g.dst_list = new_list_base[i];
+        g.n_terms = moved ? n_terms_w : 0u; g.src_term = w.term_base; g.dst_term = new_term_base[i];
+        const uint32_t span = std::max(g.n, std::max(g.n_list, g.n_terms));
+        if (span) {
+            segs.push_back(g);
+            move_blocks += (span + 255) / 256;
+        }
+        if (moved) continue;
+        if (w.n) {
+            for (uint32_t k = 0; k < w.n; k++) w.up_vel[k].w = u2f(off + k);
+            if ((rc = stage_put(ctx, ctx->pos[dst].p + off, w.up_pos.data(), w.n * sizeof(float4))) ||
+                (rc = stage_put(ctx, ctx->vel[dst].p + off, w.up_vel.data(), w.n * sizeof(float4))) ||
+                (rc = stage_put(ctx, ctx->frc[dst].p + off, w.up_frc.data(), w.n * sizeof(float4))))
                 return rc;
         }
+        tst.resize(w.n);
+        for (uint32_t b = 0; b < w.n; b++)
+            tst[b] = new_list_base[i] + (w.term_start_local.empty() ? 0u : w.term_start_local[b]);
+        if ((rc = stage_put(ctx, ctx->term_start_alt.p + off, tst.data(), tst.size() * 4))) return rc;
+        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
+        tl.resize(n_list_w);
+        for (size_t k = 0; k < tl.size(); k++) {
+            const uint32_t e = w.term_list_local[k];
+            tl[k] = (e & ~kIndexMask) | ((e & kIndexMask) + new_term_base[i]);
+        }
+        if ((rc = stage_put(ctx, ctx->bead_terms_alt.p + new_list_base[i], tl.data(), tl.size() * 4))) return rc;
+        trec.clear();
+        for (const auto *list : {&w.bonds, &w.angles})
+            for (const TermRec &r0 : *list) {
+                TermRec r = r0;
+                r.a += off; r.b += off; r.c += off;
+                r.kind = (r0.kind & 0xffu) | ((uint32_t)i << kTermWorldShift);
+                trec.push_back(r);
+            }
+        if ((rc = stage_put(ctx, ctx->terms_alt.p + new_term_base[i], trec.data(), trec.size() * sizeof(TermRec))))
+            return rc;
+    }
+    if (!segs.empty()) {
+        if ((rc = stage_put(ctx, ctx->move_segs.p, segs.data(), segs.size() * sizeof(MoveSeg)))) return rc;
+        MoveBuffers mb;
+        mb.sp = ctx->pos[src].p; mb.sv = ctx->vel[src].p; mb.sf = ctx->frc[src].p;
+        mb.dp = ctx->pos[dst].p; mb.dv = ctx->vel[dst].p; mb.df = ctx->frc[dst].p;
+        mb.ssob = ctx->slot_of_bead.p; mb.sstart = ctx->term_start.p; mb.slist = ctx->bead_terms.p;
+        mb.dsob = ctx->slot_of_bead_alt.p; mb.dstart = ctx->term_start_alt.p; mb.dlist = ctx->bead_terms_alt.p;
+        mb.sterms = ctx->terms.p; mb.dterms = ctx->terms_alt.p;
+        HIP_TRY(launch_move_worlds(ctx->move_segs.p, (uint32_t)segs.size(), move_blocks, mb, ctx->stream));
     }
     if ((rc = stage_put(ctx, ctx->term_start_alt.p + nb, tstart_end.data(), 4))) return rc;
     // per-world flags follow their worlds (removed worlds' are dropped)
@@ -1364,10 +1487,10 @@ static int rebuild_force_blocks(pdpd_ctx *ctx)
     if (ctx->dirty || ctx->n_fblocks == 0) return PDPD_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::vector<const WorldHost *> ws;
-    for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
+    std::vector<uint32_t> off;
+    for (int slot : ctx->order) off.push_back(ctx->worlds.at(slot).bead_off);
     std::vector<BeadBlock> bb, fb;
-    build_blocks(ws, ctx->band_rows, bb, fb);
+    build_blocks(ctx, ctx->order, off, bb, fb);
     if ((uint32_t)fb.size() != ctx->n_fblocks) return fail(PDPD_ERR_STATE, "force block count changed");
     HIP_TRY(hipMemcpy(ctx->fblocks.p, fb.data(), fb.size() * sizeof(BeadBlock), hipMemcpyHostToDevice));
     return PDPD_OK;
@@ -1414,6 +1537,17 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         }
         return PDPD_OK;
     }
+    if (k == "reserve_beads") {
+        uint32_t n;
+        if (parse_u32(v, 0, kMaxLayoutBeads - 1, &n))
+            return fail(PDPD_ERR_INVALID, "reserve_beads must be an integer in [0, 2^26)");
+        if (!ctx->worlds.empty() || !ctx->order.empty())
+            return fail(PDPD_ERR_STATE, "reserve_beads must be set before any world is added");
+        HIP_TRY(hipSetDevice(ctx->device));
+        if (int rc = reserve_capacity(ctx, n)) return rc;
+        ctx->reserved_beads = std::max(ctx->reserved_beads, n);
+        return PDPD_OK;
+    }
     if (k == "force_kernel") {
         uint32_t n;
         if (parse_u32(v, 0, 1, &n)) return fail(PDPD_ERR_INVALID, "force_kernel must be 0 or 1");
@@ -1431,9 +1565,7 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
             if (!ctx->dirty) {     // re-list the bricks of the current layout
                 HIP_TRY(hipSetDevice(ctx->device));
                 HIP_TRY(hipStreamSynchronize(ctx->stream));
-                std::vector<const WorldHost *> ws;
-                for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
-                if (int rc = upload_bricks(ctx, ws)) return rc;
+                if (int rc = upload_bricks(ctx)) return rc;
             }
         }
         return PDPD_OK;
@@ -1461,6 +1593,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     else if (k == "force_band_rows") v = std::to_string(ctx->band_rows);
     else if (k == "force_kernel") v = std::to_string(ctx->force_kernel);
     else if (k == "force_brick") v = std::to_string(ctx->brick_shape);
+    else if (k == "reserve_beads") v = std::to_string(ctx->reserved_beads);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());
     return PDPD_OK;
@@ -1495,6 +1628,9 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
                   std::to_string(c);
     }
     s += "force_brick=" + std::to_string(ctx->brick_shape) + "\tbrick shape in cells (x, y, z): " + shapes + "\n";
+    s += "reserve_beads=" + std::to_string(ctx->reserved_beads) +
+         "\tdevice and pinned staging capacity set aside for this many resident beads (layouts up to it are built"
+         " without allocating)\n";
     snprintf(buf, buflen, "%s", s.c_str());
     return PDPD_OK;
 }
@@ -1797,14 +1933,12 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    std::vector<const WorldHost *> ws;
-    for (int slot : ctx->order) ws.push_back(&ctx->worlds.at(slot));
     auto set_shape = [&](uint32_t kernel, uint32_t shape) -> int {
         ctx->force_kernel = kernel;
         if (shape != ctx->brick_shape) {
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             ctx->brick_shape = shape;
-            if (int r = upload_bricks(ctx, ws)) return r;
+            if (int r = upload_bricks(ctx)) return r;
         }
         return PDPD_OK;
     };

@@ -51,25 +51,30 @@ hipError_t launch_force_brick(const StepLaunch &L, hipStream_t st);
 uint32_t brick_shape_count();
 void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
-hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st);
-// Relayout helpers: a resident world's state moved to a new offset (bead index
-// rebased); a resident world's bonded-term tables moved (pointers at its first
-// entries; list starts, term indices and bead indices shifted by d_list,
-// d_term and d_bead modulo 2^32, world index replaced); per-world flags carried
-// to a new layout (nw[i] = old[map[i]], 0 where map[i] is ~0).
-hipError_t launch_move_world(const float4 *sp, const float4 *sv, const float4 *sf, float4 *dp, float4 *dv, float4 *df,
-                             uint32_t n, uint32_t old_off, uint32_t new_off, hipStream_t st);
-hipError_t launch_move_terms(const uint32_t *s_start, uint32_t *d_start, uint32_t n, uint32_t d_list,
-                             const uint32_t *s_list, uint32_t *d_lst, uint32_t n_list, uint32_t d_term,
-                             const TermRec *s_terms, TermRec *d_terms, uint32_t n_terms, uint32_t d_bead,
-                             uint32_t world, hipStream_t st);
+// Relayout: one segment per world of the new layout, in order (see
+// k_move_worlds); blocks [first_block, next segment's first_block) serve it,
+// 256 entries each of max(n, n_list, n_terms).
+struct MoveSeg {
+    uint32_t first_block, world, fresh, n;
+    uint32_t src_off, dst_off;
+    uint32_t n_list, src_list, dst_list;
+    uint32_t n_terms, src_term, dst_term;
+};
+struct MoveBuffers {
+    const float4 *sp, *sv, *sf;
+    float4 *dp, *dv, *df;
+    const uint32_t *ssob, *sstart, *slist;
+    uint32_t *dsob, *dstart, *dlist;
+    const TermRec *sterms;
+    TermRec *dterms;
+};
+hipError_t launch_move_worlds(const MoveSeg *segs, uint32_t n_segs, uint32_t n_blocks, const MoveBuffers &B,
+                              hipStream_t st);
+// Per-world flags carried to a new layout (nw[i] = old[map[i]], 0 where map[i] is ~0).
 hipError_t launch_remap_u32(uint32_t *nw, const uint32_t *old, const uint32_t *map, uint32_t n, hipStream_t st);
 // Pair records of world `world` (n_beads beads) from buffer buf, step counter t; appended at *count.
 hipError_t launch_trace_pairs(const DeviceState &S, int buf, uint32_t world, uint32_t n_beads, uint32_t t,
                               pdpd_pair_record *out, uint32_t cap, uint32_t *count, hipStream_t st);
-// slot_of_bead of a world moved from bead offset old_off to new_off (old = null: identity).
-hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
-                               hipStream_t st);
 // Per-world FP64 observables of buffer buf (momentum, sum v.v, conservative
 // virial, pair and bead counts; kObsValues doubles per world) into out, via
 // one partial per bead block in partial (n_bead_blocks x kObsValues doubles).

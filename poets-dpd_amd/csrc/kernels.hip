@@ -373,16 +373,6 @@ __global__ __launch_bounds__(kBeadThreads) void k_scatter(DeviceState S, int cur
     S.sorted_idx[j] = fbits(S.vel[cur][s].w);
 }
 
-// slot_of_bead for a world moved to bead offset new_off: from its mapping at
-// old_off (resident worlds keep their order), or the identity (worlds uploaded
-// in bead order).
-__global__ void k_rebase_slots(const uint32_t *__restrict__ old, uint32_t *__restrict__ nw, uint32_t n,
-                               uint32_t old_off, uint32_t new_off)
-{
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < n) nw[new_off + b] = old ? old[old_off + b] - old_off + new_off : new_off + b;
-}
-
 // --------------------------------------------------------- pair force ----
 
 // Shift codes of a candidate's periodic image, 2 bits per axis: 0 none,
@@ -926,44 +916,55 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
-// A resident world moved from bead offset old_off (buffers s*) to new_off
-// (buffers d*): positions, velocities (bead index rebased) and forces.
-__global__ void k_move_world(const float4 *__restrict__ sp, const float4 *__restrict__ sv,
-                             const float4 *__restrict__ sf, float4 *__restrict__ dp, float4 *__restrict__ dv,
-                             float4 *__restrict__ df, uint32_t n, uint32_t old_off, uint32_t new_off)
+// A new layout built from the previous one: one segment per world, blocks
+// dealt to segments by their first_block. A resident world's beads (state in
+// its sorted order, bead index in v.w rebased), bead-slot map entries and
+// term-list starts move from src_off to dst_off, its term-list entries and
+// term records move with their term indices and bead indices shifted and the
+// world's new index; differences are taken modulo 2^32. An uploaded world's
+// state and tables come from the host: only its bead-slot map (the identity:
+// uploaded in bead order) is set here.
+__global__ __launch_bounds__(256) void k_move_worlds(const MoveSeg *__restrict__ segs, uint32_t n_segs,
+                                                     MoveBuffers B)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    dp[new_off + b] = sp[old_off + b];
-    float4 v = sv[old_off + b];
-    v.w = __uint_as_float(__float_as_uint(v.w) - old_off + new_off);
-    dv[new_off + b] = v;
-    df[new_off + b] = sf[old_off + b];
-}
-
-// One resident world's bonded-term tables moved to a new layout (each pointer
-// at the world's first entry): its beads' term-list starts shifted by the
-// change d_list of the world's list base, its list entries' term indices by
-// the change d_term of its first term index, its term records' bead indices
-// by the change d_bead of its bead offset, with the world's new index. The
-// differences are taken modulo 2^32.
-__global__ void k_move_terms(const uint32_t *__restrict__ s_start, uint32_t *__restrict__ d_start, uint32_t n,
-                             uint32_t d_list, const uint32_t *__restrict__ s_list, uint32_t *__restrict__ d_lst,
-                             uint32_t n_list, uint32_t d_term, const TermRec *__restrict__ s_terms,
-                             TermRec *__restrict__ d_terms, uint32_t n_terms, uint32_t d_bead, uint32_t world)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d_start[i] = s_start[i] + d_list;
-    if (i < n_list) {
-        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
-        const uint32_t e = s_list[i];
-        d_lst[i] = (e & ~kIndexMask) | (((e & kIndexMask) + d_term) & kIndexMask);
+    __shared__ uint32_t s_seg;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = n_segs;                 // last segment with first_block <= blockIdx.x
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (segs[mid].first_block <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        s_seg = lo;
     }
-    if (i < n_terms) {
-        TermRec r = s_terms[i];
+    __syncthreads();
+    const MoveSeg g = segs[s_seg];
+    const uint32_t i = (blockIdx.x - g.first_block) * blockDim.x + threadIdx.x;
+    if (g.fresh) {
+        if (i < g.n) B.dsob[g.dst_off + i] = g.dst_off + i;
+        return;
+    }
+    const uint32_t d_bead = g.dst_off - g.src_off;
+    if (i < g.n) {
+        const uint32_t s = g.src_off + i, d = g.dst_off + i;
+        B.dp[d] = B.sp[s];
+        float4 v = B.sv[s];
+        v.w = __uint_as_float(__float_as_uint(v.w) + d_bead);
+        B.dv[d] = v;
+        B.df[d] = B.sf[s];
+        B.dsob[d] = B.ssob[s] + d_bead;
+        B.dstart[d] = B.sstart[s] + (g.dst_list - g.src_list);
+    }
+    if (i < g.n_list) {
+        constexpr uint32_t kIndexMask = (1u << kRoleShift) - 1u;
+        const uint32_t e = B.slist[g.src_list + i];
+        B.dlist[g.dst_list + i] = (e & ~kIndexMask) | (((e & kIndexMask) + (g.dst_term - g.src_term)) & kIndexMask);
+    }
+    if (i < g.n_terms) {
+        TermRec r = B.sterms[g.src_term + i];
         r.a += d_bead; r.b += d_bead; r.c += d_bead;
-        r.kind = (r.kind & 0xffu) | (world << kTermWorldShift);
-        d_terms[i] = r;
+        r.kind = (r.kind & 0xffu) | (g.world << kTermWorldShift);
+        B.dterms[g.dst_term + i] = r;
     }
 }
 
@@ -973,16 +974,6 @@ __global__ void k_remap_u32(uint32_t *__restrict__ nw, const uint32_t *__restric
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) nw[i] = map[i] == ~0u ? 0u : old[map[i]];
-}
-
-__global__ void k_rebase_index(float4 *__restrict__ vel, uint32_t n, uint32_t delta)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        float4 v = vel[i];
-        v.w = __uint_as_float(__float_as_uint(v.w) + delta);
-        vel[i] = v;
-    }
 }
 
 // ------------------------------------------------------------ launchers ----
@@ -1070,31 +1061,11 @@ hipError_t launch_trace_pairs(const DeviceState &S, int buf, uint32_t world, uin
     return hipGetLastError();
 }
 
-hipError_t launch_rebase_slots(const uint32_t *old, uint32_t *nw, uint32_t n, uint32_t old_off, uint32_t new_off,
-                               hipStream_t st)
+hipError_t launch_move_worlds(const MoveSeg *segs, uint32_t n_segs, uint32_t n_blocks, const MoveBuffers &B,
+                              hipStream_t st)
 {
-    if (n == 0) return hipSuccess;
-    k_rebase_slots<<<ceil_div(n, 256), 256, 0, st>>>(old, nw, n, old_off, new_off);
-    return hipGetLastError();
-}
-
-hipError_t launch_move_world(const float4 *sp, const float4 *sv, const float4 *sf, float4 *dp, float4 *dv, float4 *df,
-                             uint32_t n, uint32_t old_off, uint32_t new_off, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    k_move_world<<<ceil_div(n, 256), 256, 0, st>>>(sp, sv, sf, dp, dv, df, n, old_off, new_off);
-    return hipGetLastError();
-}
-
-hipError_t launch_move_terms(const uint32_t *s_start, uint32_t *d_start, uint32_t n, uint32_t d_list,
-                             const uint32_t *s_list, uint32_t *d_lst, uint32_t n_list, uint32_t d_term,
-                             const TermRec *s_terms, TermRec *d_terms, uint32_t n_terms, uint32_t d_bead,
-                             uint32_t world, hipStream_t st)
-{
-    const uint32_t m = std::max(n, std::max(n_list, n_terms));
-    if (m == 0) return hipSuccess;
-    k_move_terms<<<ceil_div(m, 256), 256, 0, st>>>(s_start, d_start, n, d_list, s_list, d_lst, n_list, d_term,
-                                                    s_terms, d_terms, n_terms, d_bead, world);
+    if (n_segs == 0 || n_blocks == 0) return hipSuccess;
+    k_move_worlds<<<n_blocks, 256, 0, st>>>(segs, n_segs, B);
     return hipGetLastError();
 }
 
@@ -1102,13 +1073,6 @@ hipError_t launch_remap_u32(uint32_t *nw, const uint32_t *old, const uint32_t *m
 {
     if (n == 0) return hipSuccess;
     k_remap_u32<<<ceil_div(n, 256), 256, 0, st>>>(nw, old, map, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_rebase_index(float4 *vel, uint32_t n, uint32_t delta, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    k_rebase_index<<<ceil_div(n, 256), 256, 0, st>>>(vel, n, delta);
     return hipGetLastError();
 }
 

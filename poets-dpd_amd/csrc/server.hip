@@ -146,6 +146,17 @@ int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out)
         s->ctxs.push_back(c);
     }
     s->part_beads.assign(parts, 0);
+    if (cfg->hbm_budget_bytes) {
+        // set the budget's worth of beads aside in every context (a context may
+        // hold up to twice its share), so admissions never allocate
+        const uint64_t total = cfg->hbm_budget_bytes / 112ull;
+        const uint64_t each = std::min<uint64_t>(std::min<uint64_t>(total, parts > 1 ? 2 * total / parts : total),
+                                                 (1ull << 26) - 1);
+        for (pdpd_ctx *c : s->ctxs) {
+            int rc = pdpd_set_config(c, "reserve_beads", std::to_string(each).c_str());
+            if (rc) return sfail(rc, pdpd_last_error());
+        }
+    }
     *out = s.release();
     return PDPD_OK;
 }
