@@ -548,7 +548,7 @@ int reserve_capacity(pdpd_ctx *ctx, size_t n)
     if (n == 0) return PDPD_OK;
     int rc;
     for (int b = 0; b < 2; b++)
-        if ((rc = reserve(ctx, ctx->pos[b], n)) || (rc = reserve(ctx, ctx->vel[b], n)) ||
+        if ((rc = reserve(ctx, ctx->pos[b], n + kPosPad)) || (rc = reserve(ctx, ctx->vel[b], n)) ||
             (rc = reserve(ctx, ctx->frc[b], n)))
             return rc;
     const size_t W = 256, nc = n, nt = n / 2 + 1;
@@ -681,7 +681,7 @@ int relayout(pdpd_ctx *ctx)
     // ---- device buffers of the new layout (capacity kept across relayouts)
     const int src = ctx->cur, dst = ctx->cur ^ 1;
     int rc;
-    if ((rc = reserve(ctx, ctx->pos[dst], nb)) || (rc = reserve(ctx, ctx->vel[dst], nb)) ||
+    if ((rc = reserve(ctx, ctx->pos[dst], nb + kPosPad)) || (rc = reserve(ctx, ctx->vel[dst], nb)) ||
         (rc = reserve(ctx, ctx->frc[dst], nb)) || (rc = reserve(ctx, ctx->slot_of_bead_alt, nb)) ||
         (rc = reserve(ctx, ctx->term_start_alt, nb + 1)) || (rc = reserve(ctx, ctx->bead_terms_alt, ntl)) ||
         (rc = reserve(ctx, ctx->terms_alt, nterm)) || (rc = reserve(ctx, ctx->wflags_alt, W)) ||
@@ -778,7 +778,7 @@ int relayout(pdpd_ctx *ctx)
     std::swap(ctx->bead_terms, ctx->bead_terms_alt);
     std::swap(ctx->terms, ctx->terms_alt);
     std::swap(ctx->wflags, ctx->wflags_alt);
-    if ((rc = reserve(ctx, ctx->pos[src], nb)) || (rc = reserve(ctx, ctx->vel[src], nb)) ||
+    if ((rc = reserve(ctx, ctx->pos[src], nb + kPosPad)) || (rc = reserve(ctx, ctx->vel[src], nb)) ||
         (rc = reserve(ctx, ctx->frc[src], nb)))
         return rc;
     ctx->cur = dst;
@@ -1593,6 +1593,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     else if (k == "force_band_rows") v = std::to_string(ctx->band_rows);
     else if (k == "force_kernel") v = std::to_string(ctx->force_kernel);
     else if (k == "force_brick") v = std::to_string(ctx->brick_shape);
+    else if (k == "force_brick_shapes") v = std::to_string(brick_shape_count());
     else if (k == "reserve_beads") v = std::to_string(ctx->reserved_beads);
     else return fail(PDPD_ERR_INVALID, "Unknown engine config name '" + k + "'");
     snprintf(buf, buflen, "%s", v.c_str());

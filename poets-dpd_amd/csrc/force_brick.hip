@@ -516,10 +516,19 @@ __global__ __launch_bounds__(NW * 64) void k_force_brick(DeviceState S, int nxt,
     X(4, 4, 4, 4, 8, 768, 288, 192)               \
     X(5, 4, 4, 8, 8, 1280, 512, 192)
 
-uint32_t brick_shape_count() { return 6; }
+// Shapes 0-5 are bricks (this kernel); shapes from kSlabShape0 on are the
+// slabs of force_slab.hip, one cell thick in z.
+constexpr uint32_t kSlabShape0 = 6;
+
+uint32_t brick_shape_count() { return kSlabShape0 + slab_shape_count(); }
 
 void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz)
 {
+    if (shape >= kSlabShape0) {
+        slab_shape_dims(shape - kSlabShape0, bx, by);
+        *bz = 1;
+        return;
+    }
     switch (shape) {
 #define X(id, a, b, c, nw, hcap, homecap, qcap) case id: *bx = a; *by = b; *bz = c; return;
         PDPD_BRICK_SHAPES(X)
@@ -530,6 +539,7 @@ void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz)
 
 hipError_t launch_force_brick(const StepLaunch &L, hipStream_t st)
 {
+    if (L.brick_shape >= kSlabShape0) return launch_force_slab(L, L.brick_shape - kSlabShape0, st);
     DeviceState Sp = L.S;
     Sp.ctr = L.S.ctr + L.cpar * L.n_worlds;      // counters read by that step
     const int nxt = L.cur ^ 1;

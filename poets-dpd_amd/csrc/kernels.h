@@ -50,6 +50,12 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st);
 hipError_t launch_force_brick(const StepLaunch &L, hipStream_t st);
 uint32_t brick_shape_count();
 void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz);
+// The pair-force launch over slabs of cells one cell thick in z
+// (force_slab.hip); same inputs, outputs and bits. The brick shapes from
+// kSlabShape0 on name its shapes.
+hipError_t launch_force_slab(const StepLaunch &L, uint32_t shape, hipStream_t st);
+uint32_t slab_shape_count();
+void slab_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 // Relayout: one segment per world of the new layout, in order (see
 // k_move_worlds); blocks [first_block, next segment's first_block) serve it,

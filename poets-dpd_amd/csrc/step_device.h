@@ -6,6 +6,8 @@
 #include "device_types.h"
 #include "pdpd_hip.h"
 
+#include <type_traits>
+
 namespace pdpd {
 
 constexpr float kMinR2 = 1e-9f * 1e-9f;           // MIN_DISTANCE_CUTOFF_SQR
@@ -59,6 +61,16 @@ __device__ __forceinline__ float rng_uniform(uint32_t r)
 // ------------------------------------------------------------ helpers ----
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
+
+// The pair test kMinR2 <= r2 < 1 for r2 >= 0 or NaN (a sum of squares), on the
+// bit patterns: non-negative floats order like their bits, so one subtraction
+// and one unsigned compare select the same set as the two float compares (NaN
+// and +inf fall outside).
+__device__ __forceinline__ bool in_cutoff(float r2)
+{
+    const uint32_t lo = __float_as_uint(kMinR2), span = __float_as_uint(1.0f) - lo;
+    return __float_as_uint(r2) - lo < span;
+}
 
 // Exclusive scan of one value per thread across a block of NW wave64s.
 // Returns the exclusive prefix; *total receives the block sum.
@@ -191,12 +203,9 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
-    return v;
-}
+// Maximum over the wave's active lanes (the device library's DPP reduction:
+// register moves between lanes, no LDS round trips).
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) { return __ockl_wfred_max_u32(v); }
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_sync()

@@ -140,11 +140,16 @@ int main(int argc, char **argv)
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
-        {"dense_brick3", {5, 4, 6}, 0.0, false, 0.0, 2, "force_brick=3"},
-        {"poly_brick1", {6, 7, 8}, 0.2, false, 0.0, 2, "force_brick=1"},
-        {"wall_brick2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_brick=2"},
-        {"poly_brick5", {9, 8, 10}, 0.1, false, 0.0, 2, "force_brick=5"},
-        {"poly_brick_grid", {9, 8, 10}, 0.1, false, 0.0, 3, "force_grid=3"},
+        {"dense_brick3", {5, 4, 6}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=3"},
+        {"poly_brick1", {6, 7, 8}, 0.2, false, 0.0, 2, "force_kernel=1,force_brick=1"},
+        {"wall_brick2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_kernel=1,force_brick=2"},
+        {"poly_brick5", {9, 8, 10}, 0.1, false, 0.0, 2, "force_kernel=1,force_brick=5"},
+        {"poly_brick_grid", {9, 8, 10}, 0.1, false, 0.0, 3, "force_kernel=1,force_grid=3"},
+        {"dense_slab6", {5, 4, 6}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=6"},
+        {"poly_slab7", {6, 7, 8}, 0.2, false, 0.0, 2, "force_kernel=1,force_brick=7"},
+        {"wall_slab8", {6, 5, 7}, 0.2, true, 1.0, 2, "force_kernel=1,force_brick=8"},
+        {"poly_slab9", {19, 6, 5}, 0.1, false, 0.0, 3, "force_kernel=1,force_brick=9"},
+        {"poly_slab_grid", {9, 8, 10}, 0.1, false, 0.0, 3, "force_kernel=1,force_brick=6,force_grid=3"},
         {"poly_per_slot", {6, 7, 8}, 0.1, false, 0.0, 2, "force_kernel=0"},
         {"spatial", {5, 4, 6}, 0.2, true, 0.5, 2, nullptr},      // strengths per cell, batched with a constant world
     };
@@ -166,10 +171,13 @@ int main(int argc, char **argv)
         if (why && *why) { fprintf(stderr, "unsupported: %s\n", why); return 1; }
         pdpd_ctx *ctx;
         CHECK(pdpd_ctx_create(0, &ctx));
-        if (c.cfg) {
+        for (const char *p = c.cfg; p && *p;) {     // key=value[,key=value...]
             char k[64], v[64];
-            sscanf(c.cfg, "%63[^=]=%63s", k, v);
+            int used = 0;
+            if (sscanf(p, "%63[^=]=%63[^,]%n", k, v, &used) != 2) { fprintf(stderr, "bad config\n"); return 1; }
             CHECK(pdpd_set_config(ctx, k, v));
+            p += used;
+            if (*p == ',') p++;
         }
         int slot;
         CHECK(pdpd_world_add(ctx, &w.d, &slot));

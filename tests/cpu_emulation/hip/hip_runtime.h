@@ -183,6 +183,14 @@ inline uint32_t __shfl_up(uint32_t v, unsigned d, int = 64) { return (uint32_t)e
 inline uint32_t __shfl(uint32_t v, int src, int = 64) { return (uint32_t)emu::wave_op(2, v, src); }
 inline uint32_t __shfl_xor(uint32_t v, int m, int = 64) { return (uint32_t)emu::wave_op(3, v, m); }
 inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t v) { return (uint32_t)emu::wave_op(5, v, 0); }
+inline uint32_t __ockl_wfred_max_u32(uint32_t v)
+{
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t o = __shfl_xor(v, m);
+        v = o > v ? o : v;
+    }
+    return v;
+}
 inline void __builtin_amdgcn_s_sleep(int) { emu::yield_(); }
 inline void __builtin_amdgcn_wave_barrier() { (void)emu::wave_op(4, 0, 0); }
 inline void __builtin_amdgcn_fence(int, const char *) {}   // fibers run one at a time

@@ -239,7 +239,11 @@ def test_autotune_leaves_state_unchanged(gpu_lib):
                                  dict(force_kernel=1, force_brick=2), dict(force_kernel=1, force_brick=3),
                                  dict(force_kernel=1, force_brick=4), dict(force_kernel=1, force_brick=5),
                                  dict(force_kernel=1, force_grid=64, graph_steps=6),
-                                 dict(force_kernel=1, fused_integrate=0)])
+                                 dict(force_kernel=1, fused_integrate=0),
+                                 dict(force_kernel=1, force_brick=6), dict(force_kernel=1, force_brick=7),
+                                 dict(force_kernel=1, force_brick=8), dict(force_kernel=1, force_brick=9),
+                                 dict(force_kernel=1, force_brick=6, force_grid=64, graph_steps=6),
+                                 dict(force_kernel=1, force_brick=7, fused_integrate=0)])
 def test_launch_shapes_bitwise_identical(gpu_lib, cfg):
     w = W.make_world((12, 10, 9), bonded_fraction=0.1, seed=41, a_matrix=W.random_a_matrix(np.random.default_rng(4)))
     _say(f"launch shape {cfg}")

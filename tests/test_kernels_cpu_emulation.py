@@ -79,10 +79,11 @@ for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2, 1)):
     ctx.close()
 # every brick shape of the LDS brick kernel (the dense world takes the path for
 # bricks beyond the LDS capacity), and the per-slot kernel
-for shape in range(6):
+for shape in range(int(capi.Context(0).get_config("force_brick_shapes"))):
     same = True
     for name in ("poly_6x7x8", "wall_bent6", "dense3", "lattice4"):
         ctx = capi.Context(0)
+        ctx.set_config("force_kernel", 1)
         ctx.set_config("force_brick", shape)
         slot = ctx.add(worlds[name])
         ctx.run(2)
@@ -140,7 +141,9 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
-    assert sum(l.endswith("bitwise") for l in lines) == 16, r.stdout
+    shapes = [l for l in lines if l.startswith("brick")]
+    assert len(shapes) == 10 and "DIFFERENT" not in r.stdout, r.stdout     # 6 bricks, 4 slabs
+    assert sum(l.endswith("bitwise") for l in lines) == 10 + len(shapes), r.stdout
 
 
 SERVE_PROBE = r"""
