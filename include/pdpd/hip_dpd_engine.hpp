@@ -570,16 +570,16 @@ public:
         pdpd_host::download_into(m_parts[e.part]->get(), e.cslot, *e.state);
     }
 
-    // Let the autotuner pick the launch shape (force grid, variant, steps per
-    // graph, chained steps) of a single-part engine. With several parts the
-    // default shapes are kept: the autotuner times a context alone, and the
-    // shapes it picks that way measured slower with the parts running
-    // together (256 worlds over 12 parts: 3.92e9 against 4.10e9 bead-steps/s).
+    // Let the autotuner pick the launch shape (pair-force kernel, grid,
+    // variant or brick shape, steps per graph, chained steps), timed with all
+    // parts running side by side; every part takes it.
     std::string Autotune(unsigned steps_per_trial)
     {
-        if (m_parts.size() > 1) return "default launch shapes (" + std::to_string(m_parts.size()) + " parts)";
+        std::vector<pdpd_ctx *> cs;
+        for (auto &c : m_parts) cs.push_back(c->get());
         char buf[256] = {0};
-        pdpd_host::check(pdpd_autotune(m_parts[0]->get(), steps_per_trial, buf, sizeof(buf)), "pdpd_autotune");
+        pdpd_host::check(pdpd_autotune_group(cs.data(), (uint32_t)cs.size(), steps_per_trial, buf, sizeof(buf)),
+                         "pdpd_autotune_group");
         return buf;
     }
 

@@ -314,6 +314,14 @@ int pdpd_observables_noalloc(pdpd_ctx *ctx, double *out_dev, uint32_t out_worlds
  * the worlds do not advance); the choice stays in the configuration. */
 int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len);
 
+/* The same over n contexts of one device that run side by side (an ensemble
+ * spread over several contexts, whose streams overlap): every trial enqueues
+ * the steps of all of them and is timed from the first context's start to the
+ * last one's end; every context takes the chosen shape. No counterpart in the
+ * reference, which tunes nothing (its optimisable.hpp:5-46 only sketches a
+ * tuning interface). */
+int pdpd_autotune_group(pdpd_ctx **ctxs, uint32_t n, uint32_t steps_per_trial, char *choice, size_t choice_len);
+
 /* ---- serving: continuous batching over one context (no counterpart in the
  * reference, whose engines run one world per Run) ----
  * Requests queue in arrival order. At each interval boundary the loop retires

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cfloat>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -548,7 +549,7 @@ int reserve_capacity(pdpd_ctx *ctx, size_t n)
     if (n == 0) return PDPD_OK;
     int rc;
     for (int b = 0; b < 2; b++)
-        if ((rc = reserve(ctx, ctx->pos[b], n + kPosPad)) || (rc = reserve(ctx, ctx->vel[b], n)) ||
+        if ((rc = reserve(ctx, ctx->pos[b], n)) || (rc = reserve(ctx, ctx->vel[b], n)) ||
             (rc = reserve(ctx, ctx->frc[b], n)))
             return rc;
     const size_t W = 256, nc = n, nt = n / 2 + 1;
@@ -681,7 +682,7 @@ int relayout(pdpd_ctx *ctx)
     // ---- device buffers of the new layout (capacity kept across relayouts)
     const int src = ctx->cur, dst = ctx->cur ^ 1;
     int rc;
-    if ((rc = reserve(ctx, ctx->pos[dst], nb + kPosPad)) || (rc = reserve(ctx, ctx->vel[dst], nb)) ||
+    if ((rc = reserve(ctx, ctx->pos[dst], nb)) || (rc = reserve(ctx, ctx->vel[dst], nb)) ||
         (rc = reserve(ctx, ctx->frc[dst], nb)) || (rc = reserve(ctx, ctx->slot_of_bead_alt, nb)) ||
         (rc = reserve(ctx, ctx->term_start_alt, nb + 1)) || (rc = reserve(ctx, ctx->bead_terms_alt, ntl)) ||
         (rc = reserve(ctx, ctx->terms_alt, nterm)) || (rc = reserve(ctx, ctx->wflags_alt, W)) ||
@@ -778,7 +779,7 @@ int relayout(pdpd_ctx *ctx)
     std::swap(ctx->bead_terms, ctx->bead_terms_alt);
     std::swap(ctx->terms, ctx->terms_alt);
     std::swap(ctx->wflags, ctx->wflags_alt);
-    if ((rc = reserve(ctx, ctx->pos[src], nb + kPosPad)) || (rc = reserve(ctx, ctx->vel[src], nb)) ||
+    if ((rc = reserve(ctx, ctx->pos[src], nb)) || (rc = reserve(ctx, ctx->vel[src], nb)) ||
         (rc = reserve(ctx, ctx->frc[src], nb)))
         return rc;
     ctx->cur = dst;
@@ -1883,58 +1884,91 @@ int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out)
 }
 
 // Times launch shapes that do not change results (the pair-force kernel,
-// its grid, variant or brick shape, and the graph length) on a saved copy of
-// the state, keeps the fastest.
-int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len)
+// its grid, variant or brick shape, the graph length and chained steps) on a
+// saved copy of the state, keeps the fastest. A group of contexts on one
+// device (an ensemble spread over several streams) is tuned as it runs: every
+// trial enqueues the steps of all of them, and its time is from the first
+// context's start to the last one's end; every context takes the same shape.
+static int autotune_group(const std::vector<pdpd_ctx *> &all, uint32_t steps_per_trial, char *choice,
+                          size_t choice_len)
 {
-    if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
-    HIP_TRY(hipSetDevice(ctx->device));
-    int rc = ensure_layout(ctx);
-    if (rc) return rc;
-    if (ctx->n_worlds == 0) return fail(PDPD_ERR_INVALID, "no worlds to tune on");
+    std::vector<pdpd_ctx *> cs;
+    for (pdpd_ctx *c : all) {
+        if (!c) return fail(PDPD_ERR_INVALID, "null context");
+        if (c->device != all[0]->device) return fail(PDPD_ERR_INVALID, "contexts of a group must share a device");
+        HIP_TRY(hipSetDevice(c->device));
+        if (int rc = ensure_layout(c)) return rc;
+        if (c->n_worlds) cs.push_back(c);
+    }
+    if (cs.empty()) return fail(PDPD_ERR_INVALID, "no worlds to tune on");
+    HIP_TRY(hipSetDevice(cs[0]->device));
     steps_per_trial = std::max<uint32_t>(4, steps_per_trial) & ~1u;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const size_t nb = ctx->n_beads;
-    DevBuf<float4> sp, sv, sf;
-    DevBuf<uint32_t> ss;
-    HIP_TRY(dev_alloc(sp, nb));
-    HIP_TRY(dev_alloc(sv, nb));
-    HIP_TRY(dev_alloc(sf, nb));
-    HIP_TRY(dev_alloc(ss, nb));
-    const int c0 = ctx->cur, cp0 = ctx->cpar;
-    // save and restore on the engine's stream: the stream is non-blocking, so a
-    // plain hipMemcpy would not be ordered with the trial steps
-    HIP_TRY(hipMemcpyAsync(sp.p, ctx->pos[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(sv.p, ctx->vel[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(sf.p, ctx->frc[c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ss.p, ctx->slot_of_bead.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
-    auto restore = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(ctx->pos[c0].p, sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->vel[c0].p, sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->frc[c0].p, sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->slot_of_bead.p, ss.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+    const size_t nc = cs.size();
+    struct Saved {
+        DevBuf<float4> sp, sv, sf;
+        DevBuf<uint32_t> ss;
+        int c0 = 0, cp0 = 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+    };
+    std::vector<Saved> sv(nc);
+    auto cleanup = [&]() {
+        for (Saved &q : sv) {
+            q.sp.release(); q.sv.release(); q.sf.release(); q.ss.release();
+            if (q.e0) (void)hipEventDestroy(q.e0);
+            if (q.e1) (void)hipEventDestroy(q.e1);
+            q.e0 = q.e1 = nullptr;
+        }
+    };
+    struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{cleanup};
+    for (size_t i = 0; i < nc; i++) {
+        pdpd_ctx *ctx = cs[i];
+        Saved &q = sv[i];
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const size_t nb = ctx->n_beads;
+        HIP_TRY(dev_alloc(q.sp, nb));
+        HIP_TRY(dev_alloc(q.sv, nb));
+        HIP_TRY(dev_alloc(q.sf, nb));
+        HIP_TRY(dev_alloc(q.ss, nb));
+        q.c0 = ctx->cur;
+        q.cp0 = ctx->cpar;
+        // saved and restored on the engine's stream: the stream is non-blocking,
+        // so a plain hipMemcpy would not be ordered with the trial steps
+        HIP_TRY(hipMemcpyAsync(q.sp.p, ctx->pos[q.c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(q.sv.p, ctx->vel[q.c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(q.sf.p, ctx->frc[q.c0].p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(q.ss.p, ctx->slot_of_bead.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                                ctx->stream));
-        ctx->cur = c0;
-        ctx->cpar = cp0;
+        HIP_TRY(hipEventCreate(&q.e0));
+        HIP_TRY(hipEventCreate(&q.e1));
+    }
+    auto restore = [&](size_t i) -> int {
+        pdpd_ctx *ctx = cs[i];
+        const Saved &q = sv[i];
+        const size_t nb = ctx->n_beads;
+        HIP_TRY(hipMemcpyAsync(ctx->pos[q.c0].p, q.sp.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->vel[q.c0].p, q.sv.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->frc[q.c0].p, q.sf.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->slot_of_bead.p, q.ss.p, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        ctx->cur = q.c0;
+        ctx->cpar = q.cp0;
         return PDPD_OK;
     };
     int cus = 256;
     {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        if (hipGetDeviceProperties(&prop, cs[0]->device) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
     const uint32_t grids[] = {0u, (uint32_t)cus * 4, (uint32_t)cus * 8, (uint32_t)cus * 16};
     const uint32_t graphs[] = {10u, 20u, 50u};
-    uint32_t best_grid = ctx->force_grid, best_graph = ctx->graph_steps, best_var = ctx->force_variant;
-    uint32_t best_kernel = ctx->force_kernel, best_shape = ctx->brick_shape;
-    const uint32_t fuse0 = ctx->fuse() ? 1u : 0u;
+    pdpd_ctx *c0 = cs[0];
+    uint32_t best_grid = c0->force_grid, best_graph = c0->graph_steps, best_var = c0->force_variant;
+    uint32_t best_kernel = c0->force_kernel, best_shape = c0->brick_shape;
+    const uint32_t fuse0 = c0->fuse() ? 1u : 0u;
     uint32_t best_fuse = fuse0;
     double best_ms = 1e30;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    auto set_shape = [&](uint32_t kernel, uint32_t shape) -> int {
+    auto set_shape = [&](pdpd_ctx *ctx, uint32_t kernel, uint32_t shape) -> int {
         ctx->force_kernel = kernel;
         if (shape != ctx->brick_shape) {
             HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1943,27 +1977,44 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
         }
         return PDPD_OK;
     };
-    uint32_t cur_kernel = ctx->force_kernel, cur_shape = ctx->brick_shape;
+    uint32_t cur_kernel = c0->force_kernel, cur_shape = c0->brick_shape;
     auto trial = [&](uint32_t g, uint32_t var, uint32_t gs, uint32_t fz) -> int {
-        if (int r = set_shape(cur_kernel, cur_shape)) return r;
-        ctx->force_grid = g;
-        ctx->force_variant = var;
-        ctx->graph_steps = gs;
-        ctx->fuse_mode = fz;
-        ctx->graph_version = ~0ull;
         const uint32_t steps = std::max(steps_per_trial, gs);
-        for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
-            int r;
-            if ((r = restore())) return r;
-            if ((r = load_counters(ctx, steps, nullptr))) return r;
-            if (pass) HIP_TRY(hipEventRecord(e0, ctx->stream));
-            if ((r = enqueue_steps(ctx, steps))) return r;
-            if (pass) HIP_TRY(hipEventRecord(e1, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (pdpd_ctx *ctx : cs) {
+            if (int r = set_shape(ctx, cur_kernel, cur_shape)) return r;
+            ctx->force_grid = g;
+            ctx->force_variant = var;
+            ctx->graph_steps = gs;
+            ctx->fuse_mode = fz;
+            ctx->graph_version = ~0ull;
         }
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        const double per = ms / steps;
+        double per = 0;
+        for (int pass = 0; pass < 2; pass++) {   // warm-up (graph capture), then timed
+            for (size_t i = 0; i < nc; i++) {
+                int r;
+                if ((r = restore(i))) return r;
+                if ((r = load_counters(cs[i], steps, nullptr))) return r;
+            }
+            for (size_t i = 0; i < nc; i++) {    // every context's restore done before any trial step
+                HIP_TRY(hipStreamSynchronize(cs[i]->stream));
+            }
+            for (size_t i = 0; i < nc; i++) {
+                int r;
+                if (pass) HIP_TRY(hipEventRecord(sv[i].e0, cs[i]->stream));
+                if ((r = enqueue_steps(cs[i], steps))) return r;
+                if (pass) HIP_TRY(hipEventRecord(sv[i].e1, cs[i]->stream));
+            }
+            for (size_t i = 0; i < nc; i++) HIP_TRY(hipStreamSynchronize(cs[i]->stream));
+            if (pass) {
+                float end = 0;
+                for (size_t i = 0; i < nc; i++) {
+                    float ms = 0;
+                    HIP_TRY(hipEventElapsedTime(&ms, sv[0].e0, sv[i].e1));
+                    end = std::max(end, ms);
+                }
+                per = end / steps;
+            }
+        }
         if (per < best_ms) {
             best_ms = per; best_grid = g; best_var = var; best_graph = gs; best_fuse = fz;
             best_kernel = cur_kernel; best_shape = cur_shape;
@@ -1971,17 +2022,20 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
         return PDPD_OK;
     };
     // pair-force kernels: the per-slot kernel over grid x variant, the brick
-    // kernel over its shapes (one workgroup per brick), at 20 steps per graph;
-    // then the graph length with chained steps off and on
+    // and slab kernels over their shapes (one workgroup per brick), at 20
+    // steps per graph; then the graph length with chained steps off and on
+    bool spatial = false;
+    for (pdpd_ctx *ctx : cs) spatial |= ctx->spatial;
+    int rc;
     cur_kernel = 0;
     for (uint32_t g : grids)
         for (uint32_t var = 0; var < kForceVariants; var++)
             if ((rc = trial(g, var, 20u, fuse0))) return rc;
-    if (!ctx->spatial) {
+    if (!spatial) {
         cur_kernel = 1;
         for (uint32_t sh = 0; sh < brick_shape_count(); sh++) {
             cur_shape = sh;
-            if ((rc = trial(0u, ctx->force_variant, 20u, fuse0))) return rc;
+            if ((rc = trial(0u, c0->force_variant, 20u, fuse0))) return rc;
         }
     }
     cur_kernel = best_kernel;
@@ -1990,25 +2044,52 @@ int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t 
     for (uint32_t gs : graphs)
         for (uint32_t fz = 0; fz < 2; fz++)
             if ((gs != 20u || fz != fuse0) && (rc = trial(g_best, v_best, gs, fz))) return rc;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if ((rc = restore())) return rc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    sp.release(); sv.release(); sf.release(); ss.release();
-    ctx->steps_on_layout = 0;      // the cell lists are the last trial's, not the restored state's
-    if ((rc = set_shape(best_kernel, best_shape))) return rc;
-    ctx->force_grid = best_grid;
-    ctx->force_variant = best_var;
-    ctx->graph_steps = best_graph;
-    ctx->fuse_mode = best_fuse;
-    ctx->graph_version = ~0ull;
-    if ((rc = check_flags(ctx))) return rc;
+    for (size_t i = 0; i < nc; i++)
+        if ((rc = restore(i))) return rc;
+    for (pdpd_ctx *ctx : cs) HIP_TRY(hipStreamSynchronize(ctx->stream));
+    cleanup();
+    for (pdpd_ctx *ctx : cs) {
+        ctx->steps_on_layout = 0;      // the cell lists are the last trial's, not the restored state's
+        if ((rc = set_shape(ctx, best_kernel, best_shape))) return rc;
+        ctx->force_grid = best_grid;
+        ctx->force_variant = best_var;
+        ctx->graph_steps = best_graph;
+        ctx->fuse_mode = best_fuse;
+        ctx->graph_version = ~0ull;
+        if ((rc = check_flags(ctx))) return rc;
+    }
+    // empty contexts of the group take the shape too
+    for (pdpd_ctx *ctx : all) {
+        if (ctx->n_worlds) continue;
+        ctx->force_kernel = best_kernel;
+        if (ctx->brick_shape != best_shape) {
+            ctx->brick_shape = best_shape;
+            if (!ctx->dirty && (rc = upload_bricks(ctx))) return rc;
+        }
+        ctx->force_grid = best_grid;
+        ctx->force_variant = best_var;
+        ctx->graph_steps = best_graph;
+        ctx->fuse_mode = best_fuse;
+        ctx->graph_version = ~0ull;
+    }
     if (choice && choice_len)
         snprintf(choice, choice_len,
                  "force_kernel=%u force_brick=%u force_grid=%u force_variant=%u graph_steps=%u fused_integrate=%u "
-                 "ms_per_step=%.5f",
-                 best_kernel, best_shape, best_grid, best_var, best_graph, best_fuse, best_ms);
+                 "ms_per_step=%.5f contexts=%zu",
+                 best_kernel, best_shape, best_grid, best_var, best_graph, best_fuse, best_ms, all.size());
     return PDPD_OK;
+}
+
+int pdpd_autotune(pdpd_ctx *ctx, uint32_t steps_per_trial, char *choice, size_t choice_len)
+{
+    if (!ctx) return fail(PDPD_ERR_INVALID, "null context");
+    return autotune_group({ctx}, steps_per_trial, choice, choice_len);
+}
+
+int pdpd_autotune_group(pdpd_ctx **ctxs, uint32_t n, uint32_t steps_per_trial, char *choice, size_t choice_len)
+{
+    if (!ctxs || n == 0) return fail(PDPD_ERR_INVALID, "no contexts");
+    return autotune_group(std::vector<pdpd_ctx *>(ctxs, ctxs + n), steps_per_trial, choice, choice_len);
 }
 
 int pdpd_debug_rng(int device, uint32_t n, const uint64_t *t_hash, const uint32_t *a, const uint32_t *b,

@@ -16,7 +16,6 @@ constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 2;
 constexpr int kScanBlock = kScanThreads * kScanPerThread;   // cells per scan tile
 constexpr uint32_t kMaxLayoutBeads = 1u << 26;              // pair-list entries carry 26-bit slots
-constexpr uint32_t kPosPad = 8;        // position buffers hold this many slots past the layout (candidate loads read ahead)
 
 // Per-world parameters, read by every kernel through the world index.
 struct WorldDev {

@@ -137,6 +137,7 @@ SIGNATURES = {
     "pdpd_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "pdpd_run_profiled": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pdpd_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_size_t]),
+    "pdpd_autotune_group": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t]),
     "pdpd_get_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "pdpd_world_observables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Observables)]),
     "pdpd_observables_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
@@ -490,14 +491,17 @@ class Ensemble:
         return self.parts[0].get_config(key)
 
     def autotune(self, steps_per_trial=20):
-        """The launch shape of a single-context ensemble. With several contexts
-        the default shapes are kept: the autotuner times a context alone, and
-        shapes it picks that way measured slower with the contexts running
-        together (256 worlds over 12 contexts: 3.92e9 against 4.10e9
-        bead-steps/s)."""
+        """The launch shape of the ensemble, timed as it runs: every trial
+        advances all contexts side by side (pdpd_autotune_group), and every
+        context takes the fastest shape."""
         if len(self.parts) == 1:
             return self.parts[0].autotune(steps_per_trial)
-        return f"default launch shapes ({len(self.parts)} contexts)"
+        lib = self.parts[0].lib
+        arr = (C.c_void_p * len(self.parts))(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h
+                                                for c in self.parts])
+        buf = C.create_string_buffer(256)
+        check(lib.pdpd_autotune_group(arr, len(self.parts), int(steps_per_trial), buf, 256))
+        return buf.value.decode()
 
     def close(self):
         for c in self.parts:

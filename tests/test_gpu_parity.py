@@ -421,3 +421,34 @@ def test_ensemble_over_contexts_bitwise_equal_to_solo(gpu_lib):
         assert got[i][3] == ref[3]
         for a, b in zip(got[i][:3], ref[:3]):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_ensemble_autotune_over_contexts_leaves_state_unchanged(gpu_lib):
+    """The group autotuner times every trial with all contexts of an ensemble
+    running side by side and restores every world: the trajectories after it
+    are the untuned ones bit for bit, every context takes the chosen shape, and
+    an empty context (more parts than worlds) takes it too."""
+    rng = np.random.default_rng(9)
+    worlds = [W.make_world(tuple(int(v) for v in rng.integers(5, 12, 3)), bonded_fraction=0.15, seed=120 + i,
+                           a_matrix=W.random_a_matrix(rng)) for i in range(5)]
+    _say("ensemble of 5 worlds over 6 contexts (one empty), autotune between steps 4 and 10")
+    ens = capi.Ensemble(0, parts=6)
+    slots = [ens.add(w) for w in worlds]
+    ens.run(4)
+    ens.sync()
+    choice = ens.autotune(4)
+    assert "contexts=6" in choice, choice
+    keys = ("force_kernel", "force_brick", "force_grid", "force_variant", "graph_steps", "fused_integrate")
+    cfgs = {tuple(c.get_config(k) for k in keys) for c in ens.parts}
+    assert len(cfgs) == 1, cfgs
+    ens.run(6)
+    ens.sync()
+    got = [ens.download(s) for s in slots]
+    ens.close()
+    for i, w in enumerate(worlds):
+        c, s = run_gpu(w, 10)
+        ref = c.download(s)
+        c.close()
+        assert got[i][3] == ref[3] == 10
+        for a, b in zip(got[i][:3], ref[:3]):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
