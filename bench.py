@@ -164,12 +164,16 @@ def kernel_profile(contexts, steps):
     return acc
 
 
-def roofline_of(n_beads, prof, step_ms, traffic):
-    force_ms = prof["k_force_ms"]
-    achieved = n_beads * FORCE_BYTES_PER_BEAD / (force_ms * 1e-3) / 1e9
+def roofline_of(n_beads, launches, prof, step_ms, traffic):
+    """The pair-force kernel: one launch per context per step, each context's
+    launches timed alone (kernel_profile). Per launch: mean algorithmic bytes
+    (40 B per bead) over mean duration."""
+    force_ms = prof["k_force_ms"] / launches
+    bytes_per_launch = n_beads * FORCE_BYTES_PER_BEAD / launches
+    achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_force", "kernel_ms": force_ms,
-            "bytes_per_launch": n_beads * FORCE_BYTES_PER_BEAD,
+            "launches_per_step": launches, "bytes_per_launch": bytes_per_launch,
             "valu_fp32_frac": n_beads * FLOP_PER_BEAD_STEP / (step_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
             "step_hbm_frac": n_beads * STEP_BYTES_PER_BEAD / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "kernel_ms_per_step": {k[2:-3]: prof[k] for k in KERNELS}}
@@ -208,9 +212,13 @@ def run_plain(capi, world, steps, warmup, profile_steps, traffic_name):
     step_ms = el * 1e3 / steps
     out = {"value": world.n_beads * steps / el, "unit": "bead-steps/s", "beads": world.n_beads, "steps": steps,
            "warmup": w_done, "ms_per_step": step_ms,
-           "roofline": roofline_of(world.n_beads, prof, step_ms, traffic_from_profiles(traffic_name))}
+           "roofline": roofline_of(world.n_beads, 1, prof, step_ms, traffic_from_profiles(traffic_name))}
     ctx.close()
     return out
+
+
+def active_parts(ens):
+    return sum(1 for c in ens.parts if c.worlds)
 
 
 def run_served(args, capi, W):
@@ -288,8 +296,9 @@ def main():
     ap.add_argument("--serve", type=int, default=256, help="served sweep with this many requests (0 = skip)")
     ap.add_argument("--serve-budget-gib", type=float, default=0.5)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
-    ap.add_argument("--only", choices=("ensemble", "polymer32", "large128", "served"), default=None,
-                    help="run one workload only (profiling runs)")
+    ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "polymer32", "large128", "served"),
+                    default=None, help="run one workload only (profiling runs); ensemble-profile runs only the "
+                                       "headline's profiled pass (no warm-up, no timed steps)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured hipGraph (0 = direct launches, for per-dispatch counters)")
     ap.add_argument("--config", action="append", default=[], metavar="KEY=VALUE",
@@ -341,6 +350,13 @@ def main():
         ens.set_config(*kv.split("=", 1))
         tuning += f" {kv}"
     log(f"bench: tuning: {tuning}")
+    if args.only == "ensemble-profile":
+        prof = kernel_profile(ens.parts, args.profile_steps)
+        n = active_parts(ens)
+        log(f"bench: ensemble profiled pass: {n} contexts, k_force {prof['k_force_ms']:.4f} ms per step, "
+            f"{prof['k_force_ms'] / n:.4f} ms per launch")
+        ens.close()
+        return
     if args.only == "ensemble":
         warm(ens, args.warmup)
         el = timed(ens, args.steps)
@@ -353,7 +369,7 @@ def main():
     step_ms = el * 1e3 / args.steps
     log(f"bench: {value:.4e} bead-steps/s ({step_ms:.4f} ms/step)")
     prof = kernel_profile(ens.parts, args.profile_steps)
-    roofline = roofline_of(total, prof, step_ms, traffic_from_profiles("pair_force_ensemble256_pmc.md"))
+    roofline = roofline_of(total, active_parts(ens), prof, step_ms, traffic_from_profiles("pair_force_ensemble256_pmc.md"))
 
     out = {"metric": METRIC, "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": w_done, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
