@@ -64,6 +64,12 @@ constexpr uint32_t kMaxBox = 1024;
 // faster at 98k beads (46.1 against 47.5 us per step) and slower at 6.3M
 // (1.77 against 1.74 ms): on by default up to this many resident beads.
 constexpr uint32_t kFuseAutoBeads = 1u << 20;
+// force_variant=auto: variant 4 (smaller pair queue, fewer registers, 6 waves
+// per SIMD) from this many resident beads, variant 0 below (measured: 128^3
+// world 1.19 against 1.34 ms per pair-force launch, 256-world ensemble over 12
+// contexts of 0.5M beads +8 %; 32^3 and 16^3 worlds 1 us slower)
+constexpr uint32_t kVariantAuto = UINT32_MAX;
+constexpr uint32_t kVariantLargeBeads = 1u << 18;
 constexpr uint64_t kBandMinCells = 64 * 64;   // y-z plane size from which force blocks are listed in bands
 constexpr uint32_t kBandRows = 16;            // default band height (y rows)
 constexpr uint32_t kMaxPolymerLength = 127;
@@ -227,7 +233,7 @@ struct pdpd_ctx {
     // configuration
     uint32_t graph_steps = 20;
     uint32_t force_grid = 0;
-    uint32_t force_variant = 0;
+    uint32_t force_variant = kVariantAuto;   // pair-force kernel variant, or kVariantAuto (by size)
     uint32_t lookback_tiles = 256;
     uint32_t fuse_mode = 2;               // chained steps: 0 off, 1 on, 2 on up to kFuseAutoBeads resident beads
     uint32_t band_rows = kBandRows;       // force-block order of large worlds (0 = layout order)
@@ -235,6 +241,10 @@ struct pdpd_ctx {
     uint32_t brick_shape = 0;             // brick shape index (brick_shape_dims)
 
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
+    uint32_t variant() const
+    {
+        return force_variant != kVariantAuto ? force_variant : (n_beads >= kVariantLargeBeads ? 4u : 0u);
+    }
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -302,7 +312,7 @@ struct pdpd_ctx {
         L.n_stationary = n_stationary;
         L.n_terms = n_terms;
         L.force_grid = force_grid;
-        L.force_variant = force_variant;
+        L.force_variant = variant();
         L.tile_sum = tile_sum.p;
         L.lookback_tiles = lookback_tiles;
         L.spatial = spatial ? 1u : 0u;
@@ -1572,9 +1582,9 @@ int pdpd_set_config(pdpd_ctx *ctx, const char *key, const char *value)
         return PDPD_OK;
     }
     if (k == "force_variant") {
-        uint32_t n;
-        if (parse_u32(v, 0, kForceVariants - 1, &n))
-            return fail(PDPD_ERR_INVALID, "force_variant must be an integer in [0, " + std::to_string(kForceVariants - 1) + "]");
+        uint32_t n = kVariantAuto;
+        if (v != "auto" && parse_u32(v, 0, kForceVariants - 1, &n))
+            return fail(PDPD_ERR_INVALID, "force_variant must be 'auto' or an integer in [0, " + std::to_string(kForceVariants - 1) + "]");
         ctx->force_variant = n;
         ctx->graph_version = ~0ull;
         return PDPD_OK;
@@ -1588,7 +1598,7 @@ int pdpd_get_config(pdpd_ctx *ctx, const char *key, char *buf, size_t buflen)
     std::string k(key), v;
     if (k == "graph_steps") v = std::to_string(ctx->graph_steps);
     else if (k == "force_grid") v = std::to_string(ctx->force_grid);
-    else if (k == "force_variant") v = std::to_string(ctx->force_variant);
+    else if (k == "force_variant") v = ctx->force_variant == kVariantAuto ? "auto" : std::to_string(ctx->force_variant);
     else if (k == "scan_lookback_tiles") v = std::to_string(ctx->lookback_tiles);
     else if (k == "fused_integrate") v = std::to_string(ctx->fuse_mode);
     else if (k == "force_band_rows") v = std::to_string(ctx->band_rows);
@@ -1610,9 +1620,12 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
          "\tpair-force grid for a persistent grid-stride launch (0 = one workgroup per 64 beads)\n";
     s += "scan_lookback_tiles=" + std::to_string(ctx->lookback_tiles) +
          "\tcell scan: single pass with look-back up to this many 512-cell tiles, three phases above\n";
-    s += "force_variant=" + std::to_string(ctx->force_variant) +
+    s += "force_variant=" + (ctx->force_variant == kVariantAuto ? std::string("auto") : std::to_string(ctx->force_variant)) +
          "\tpair-force kernel variant: 0 = 3 candidate loads in flight, bonded sums after the middle plane's pairs;"
-         " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave\n";
+         " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave;"
+         " 4 = 2 loads, middle plane, 128-pair queues, 6 waves per SIMD; 5 = 3 loads, 192-pair queues, 6 waves;"
+         " 6 = 2 loads, fourth wave, 128-pair queues, 6 waves; auto = 4 from " + std::to_string(kVariantLargeBeads) +
+         " resident beads, else 0\n";
     s += "fused_integrate=" + std::to_string(ctx->fuse_mode) +
          "\tchained steps, the pair-force kernel integrating the next step: 0 = off, 1 = on, 2 = on up to " +
          std::to_string(kFuseAutoBeads) + " resident beads\n";
@@ -2072,11 +2085,13 @@ static int autotune_group(const std::vector<pdpd_ctx *> &all, uint32_t steps_per
         ctx->fuse_mode = best_fuse;
         ctx->graph_version = ~0ull;
     }
-    if (choice && choice_len)
+    if (choice && choice_len) {
+        const std::string var = best_var == kVariantAuto ? std::string("auto") : std::to_string(best_var);
         snprintf(choice, choice_len,
-                 "force_kernel=%u force_brick=%u force_grid=%u force_variant=%u graph_steps=%u fused_integrate=%u "
+                 "force_kernel=%u force_brick=%u force_grid=%u force_variant=%s graph_steps=%u fused_integrate=%u "
                  "ms_per_step=%.5f contexts=%zu",
-                 best_kernel, best_shape, best_grid, best_var, best_graph, best_fuse, best_ms, all.size());
+                 best_kernel, best_shape, best_grid, var.c_str(), best_graph, best_fuse, best_ms, all.size());
+    }
     return PDPD_OK;
 }
 

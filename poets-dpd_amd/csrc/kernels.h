@@ -32,7 +32,7 @@ struct StepLaunch {
     const BrickRec *bricks;
 };
 
-constexpr uint32_t kForceVariants = 4;
+constexpr uint32_t kForceVariants = 7;
 
 // Events recorded between the kernels of one step (profiling runs only):
 // [0] integrate [1] scan [2] scatter [3] reorder [4] force (+ bonded terms) [5] stationary [6]

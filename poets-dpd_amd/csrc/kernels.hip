@@ -382,7 +382,7 @@ __device__ __forceinline__ float shift_of(uint32_t code, float L)
     return code == 1u ? L : (code == 2u ? -L : 0.0f);
 }
 
-constexpr int kQueue = 256;       // accepted pairs buffered per pair wave before they are evaluated
+constexpr int kQueue = 256;       // accepted pairs buffered per pair wave before they are evaluated (at most)
 constexpr int kLaneList = 16;     // accepted pairs buffered per home bead (queue positions, 8 bits)
 
 // Slots are below 2^26, so the byte offset of a float4 fits 32 bits: the loads
@@ -433,21 +433,21 @@ __device__ __forceinline__ void bonded_terms(const DeviceState &S, const float4 
 // values at the two beads' cells (naive_dpd_engine.hpp:452-468): a = (a_h +
 // a_o) / 2, sqrt(gamma) = sqrtf((g_h + g_o) / 2), index home type * n_types +
 // other type; other worlds use the constant tables as without it.
-template <int kScanWidth, int kWaves, bool kBondWave, bool kSpatial>
+template <int kScanWidth, int kWaves, bool kBondWave, bool kSpatial, int kQ = kQueue>
 __global__ __launch_bounds__(kBondWave ? kForceThreads : kForceBeads * kForcePlanes)
 __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt, uint32_t n_blocks,
                                                            const NextBuffers *next)
 {
     // a batch accepts at most 64 * kScanWidth pairs, which an emptied queue must hold
-    static_assert(kQueue >= kForceBeads * kScanWidth && kQueue <= 256, "queue positions are 8-bit");
+    static_assert(kQ >= kForceBeads * kScanWidth && kQ <= kQueue, "queue positions are 8-bit");
     static_assert(kLaneList >= kScanWidth, "a lane lists up to kScanWidth pairs per batch");
     constexpr int kThreads = kBondWave ? kForceThreads : kForceBeads * kForcePlanes;
     constexpr int kPairLanes = kForceBeads * kForcePlanes;
     // per pair wave: queued pairs (candidate slot | image code bits 26..31, and
     // home lane) and their evaluated (unit vector, force magnitude)
-    __shared__ uint32_t s_qslot[kForcePlanes][kQueue];
-    __shared__ uint8_t s_qhome[kForcePlanes][kQueue];
-    __shared__ float4 s_qres[kForcePlanes][kQueue];
+    __shared__ uint32_t s_qslot[kForcePlanes][kQ];
+    __shared__ uint8_t s_qhome[kForcePlanes][kQ];
+    __shared__ float4 s_qres[kForcePlanes][kQ];
     // per home bead and plane: its queued pairs in acceptance order
     __shared__ uint8_t s_list[kLaneList][kPairLanes];
     __shared__ float4 s_hp[kForceBeads], s_hv[kForceBeads];
@@ -652,7 +652,7 @@ __attribute__((amdgpu_waves_per_eu(kWaves))) void k_force(DeviceState S, int nxt
                     }
                     if (total == 0) continue;
                     // room for this batch: a lane may list up to kScanWidth more pairs
-                    if (queued + total > (uint32_t)kQueue ||
+                    if (queued + total > (uint32_t)kQ ||
                         __builtin_amdgcn_ballot_w64(listed + kScanWidth > (uint32_t)kLaneList))
                         flush();
                     uint32_t at = queued;
@@ -1040,6 +1040,11 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
         case 1: k_force<4, 4, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         case 2: k_force<2, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         case 3: k_force<3, 5, true, false><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        // smaller queues, fewer registers per lane, more waves per SIMD (the
+        // queue and register budget of variant 4 give 6 waves per SIMD against 5)
+        case 4: k_force<2, 6, false, false, 128><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        case 5: k_force<3, 6, false, false, 192><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        case 6: k_force<2, 6, true, false, 128><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         default: k_force<3, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         }
     }

@@ -138,6 +138,9 @@ int main(int argc, char **argv)
         {"poly_v1", {6, 7, 8}, 0.1, false, 0.0, 2, "force_variant=1"},
         {"wall_v2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=2"},
         {"wall_v3", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=3"},
+        {"poly_v4", {6, 7, 8}, 0.1, false, 0.0, 2, "force_variant=4"},
+        {"wall_v5", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=5"},
+        {"wall_v6", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=6,force_grid=4"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
         {"dense_brick3", {5, 4, 6}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=3"},

@@ -233,6 +233,8 @@ def test_autotune_leaves_state_unchanged(gpu_lib):
                                  dict(force_grid=1024, graph_steps=6),
                                  dict(scan_lookback_tiles=0), dict(scan_lookback_tiles=0, graph_steps=4),
                                  dict(force_variant=1), dict(force_variant=2), dict(force_variant=3),
+                                 dict(force_variant=4), dict(force_variant=5), dict(force_variant=6),
+                                 dict(force_variant=4, force_grid=64, graph_steps=6),
                                  dict(fused_integrate=0), dict(fused_integrate=1, graph_steps=6),
                                  dict(fused_integrate=1, graph_steps=0, force_grid=64),
                                  dict(force_kernel=1), dict(force_kernel=1, force_brick=1),

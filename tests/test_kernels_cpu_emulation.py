@@ -64,7 +64,7 @@ sp.set_expression("conservative", 0, 1, "30*uy+x/4")
 sp.set_expression("dissipative", 1, 1, "3+3*uz")
 sp.set_expression("dissipative", 0, 2, "4.5*(1+z/10)")
 worlds["spatial"] = sp                                          # strengths per cell
-for (name, w), var in zip(worlds.items(), (0, 1, 2, 3, 2, 1)):
+for (name, w), var in zip(worlds.items(), (4, 1, 5, 3, 6, 2)):
     ctx = capi.Context(0)
     ctx.set_config("force_variant", var)
     ctx.set_config("scan_lookback_tiles", (var & 1) * 64)     # both cell-scan forms
