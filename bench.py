@@ -17,7 +17,9 @@ Extra keys:
                 step runs them); traffic = HBM bytes per launch from the
                 committed rocprofv3 --pmc summary of the same workload
   batched       the same ensemble for 5000 steps, against the same 256 worlds
-                run one at a time through the plain engine for 5000 steps each
+                run one at a time through the plain engine for 5000 steps each;
+                one_context: the ensemble in a single context (one launch set
+                per step for all 256 worlds), autotuned, 5000 steps
   plain_polymer32 / plain_large128
                 the plain engine on the polymer world (32^3 cells, 98,304
                 beads, 10% bonded) and the large world (128^3 cells, 6.3M
@@ -290,6 +292,8 @@ def main():
     ap.add_argument("--batch-parts", type=int, default=0,
                     help="contexts (streams) the ensemble is spread over (0 = the engine's suggestion)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the autotuner's")
+    ap.add_argument("--no-one-context", action="store_true",
+                    help="skip the batched run of the ensemble in a single context")
     ap.add_argument("--plain-steps", type=int, default=1000)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine polymer and large worlds")
     ap.add_argument("--no-large", action="store_true", help="skip the plain-engine large world")
@@ -391,6 +395,19 @@ def main():
                           "speedup": batched / solo, "contexts": parts, "tuning": tuning}
         log(f"bench: batched {batched:.4e}, one at a time {solo:.4e} -> {batched / solo:.2f}x")
     ens.close()
+    if args.batch_steps and not args.no_one_context:
+        # the same ensemble in one context (one launch set per step for all 256 worlds)
+        one = capi.Ensemble(0, 1)
+        for w in worlds:
+            one.add(w)
+        tuning1 = "default launch shape" if args.no_autotune else one.autotune(10)
+        warm(one, args.warmup)
+        t1 = timed(one, args.batch_steps)
+        v1 = total * args.batch_steps / t1
+        out["batched"]["one_context"] = {"value": v1, "seconds": t1, "speedup": v1 / out["batched"]["one_at_a_time"],
+                                         "tuning": tuning1}
+        log(f"bench: batched in one context {v1:.4e} -> {v1 / out['batched']['one_at_a_time']:.2f}x")
+        one.close()
 
     if not args.no_plain:
         log("bench: plain engine, polymer world 32^3")
