@@ -26,7 +26,8 @@ Extra keys:
                 beads, 5% bonded), with per-kernel times
   served        served sweep: Poisson arrivals (50/s), cubic sides
                 log-uniform in [8, 48], 1k..20k steps, interval callbacks every
-                500 steps, an HBM budget that holds only part of the requests;
+                500 steps (each request's FP64 observables, reduced on the
+                device), an HBM budget that holds only part of the requests;
                 p50/p99 latency, admission wait and aggregate bead-steps/s
   cpu_baseline  the reference's NaiveDPDEngineHalfMerge compiled from its
                 sources (-O3 -ffast-math), one thread, bounded sample of the
@@ -230,11 +231,14 @@ def run_served(args, capi, W):
     srv = capi.Server(device=0, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), interval_steps=500,
                       time_scale=1.0, contexts=args.serve_contexts)
     seen = [0]
+    temps = []
 
-    def on_interval(rid, done, t, x, v, f):
+    def on_interval(rid, done, obs):
         seen[0] += 1
+        temps.append(obs["temperature"])
         return True
-    srv.set_callback(on_interval, copy=False)
+    # per-interval statistics from sums reduced on the device (no state copy)
+    srv.set_observables_callback(on_interval)
     ids = []
     for r in reqs:
         w = W.make_world(r["L"], bonded_fraction=r["bonded_fraction"], seed=r["seed"], a_matrix=r["a_matrix"])
@@ -258,7 +262,9 @@ def run_served(args, capi, W):
             "value": st["bead_steps_per_s"], "unit": "bead-steps/s", "makespan_s": st["makespan_s"],
             "max_resident": st["max_resident"], "max_resident_beads": st["max_resident_beads"],
             "relayouts": st["relayouts"], "relayout_mean_ms": 1e3 * st["relayout_mean_s"],
-            "relayout_max_ms": 1e3 * st["relayout_max_s"], "callbacks": seen[0], "interval_steps": 500,
+            "relayout_max_ms": 1e3 * st["relayout_max_s"], "callbacks": seen[0],
+            "callback": "observables (FP64 momentum, temperature, pressure per interval, reduced on the device)",
+            "mean_interval_temperature": float(sum(temps) / len(temps)) if temps else None, "interval_steps": 500,
             "arrival_rate_per_s": 50.0, "last_arrival_s": reqs[-1]["arrival"] if reqs else 0.0,
             "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
 

@@ -309,6 +309,19 @@ int pdpd_world_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out);
  * that overlap (other contexts' streams, other threads) race. */
 int pdpd_observables_noalloc(pdpd_ctx *ctx, double *out_dev, uint32_t out_worlds);
 
+/* pdpd_snapshot plus the FP64 observables (as pdpd_get_observables) of every
+ * world in that snapshot, reduced on the device behind the enqueued steps;
+ * returns without waiting, and steps may be enqueued behind it.
+ * pdpd_world_snapshot_observables waits for those sums only and copies one
+ * world's (a few dozen bytes, no copy of its state), or fails with
+ * PDPD_ERR_STATE, naming the flags, when that world had raised a failure
+ * flag by then. For per-interval statistics of resident worlds (the serving
+ * loop's observables callback). No counterpart in the reference, whose
+ * statistics engine (stats_dpd_engine_half_step.hpp:539-572) reduces on the
+ * host after every step. */
+int pdpd_snapshot_observables(pdpd_ctx *ctx);
+int pdpd_world_snapshot_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out);
+
 /* Pick the launch shape (pair-force grid, graph length) for the resident worlds by timing
  * candidates for steps_per_trial steps each (state is saved and restored, so
  * the worlds do not advance); the choice stays in the configuration. */
@@ -370,6 +383,14 @@ int pdpd_server_create(const pdpd_server_config *cfg, pdpd_server **out);
 int pdpd_server_destroy(pdpd_server *s);
 int pdpd_server_set_config(pdpd_server *s, const char *key, const char *value);
 int pdpd_server_set_callback(pdpd_server *s, pdpd_interval_fn fn, void *user);
+/* Per-interval observables instead of (or beside) the state: fn(user,
+ * request, steps_done, observables) after every interval of every resident
+ * request, the FP64 sums reduced on the device from the interval's snapshot
+ * (pdpd_snapshot_observables), so a request's state is copied to the host
+ * only when it completes or when a state callback is also set. Returning 0
+ * cancels the request, as for the state callback. NULL removes it. */
+typedef int (*pdpd_observables_fn)(void *user, uint64_t request, uint32_t steps_done, const pdpd_observables *obs);
+int pdpd_server_set_observables_callback(pdpd_server *s, pdpd_observables_fn fn, void *user);
 /* Queue a request; the world is copied. */
 int pdpd_server_submit(pdpd_server *s, const pdpd_world_desc *w, double arrival_s, uint32_t steps, uint64_t *id_out);
 /* Serve until every queued request has completed (arrivals against the wall clock). */

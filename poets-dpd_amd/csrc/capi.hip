@@ -219,6 +219,13 @@ struct pdpd_ctx {
     hipEvent_t snap_event = nullptr;
     DevBuf<float4> snap;             // [pos | vel | frc], snap_beads each
     DevBuf<uint32_t> snap_flags;
+    // FP64 observables of the snapshot's state (pdpd_snapshot_observables):
+    // per-block partials and per-world sums on the device, their event, and
+    // pinned host room for one world's sums and flags
+    DevBuf<double> snap_obs_partial, snap_obs;
+    hipEvent_t snap_obs_event = nullptr;
+    bool snap_has_obs = false;
+    double *snap_obs_host = nullptr;
     uint32_t snap_beads = 0;
     std::map<int, SnapWorld> snap_worlds;
     float4 *snap_host = nullptr;
@@ -277,6 +284,9 @@ struct pdpd_ctx {
         bricks.release();
         obs_first.release(); obs_partial.release(); obs_out.release();
         snap.release(); snap_flags.release();
+        snap_obs_partial.release(); snap_obs.release();
+        if (snap_obs_event) (void)hipEventDestroy(snap_obs_event);
+        if (snap_obs_host) (void)hipHostFree(snap_obs_host);
         if (snap_host) (void)hipHostFree(snap_host);
         if (snap_event) (void)hipEventDestroy(snap_event);
         if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -1378,6 +1388,7 @@ int pdpd_snapshot(pdpd_ctx *ctx)
         HIP_TRY(hipMemcpyAsync(ctx->snap_flags.p + 1, ctx->wflags.p, ctx->n_worlds * sizeof(uint32_t),
                                hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->snap_event, ctx->stream));
+    ctx->snap_has_obs = false;
     ctx->snap_beads = (uint32_t)n;
     ctx->snap_worlds.clear();
     for (uint32_t i = 0; i < ctx->order.size(); i++) {      // worlds removed since the layout was built are skipped
@@ -1434,6 +1445,60 @@ int pdpd_world_snapshot_download(pdpd_ctx *ctx, int slot, double *x, double *v, 
         }
     }
     if (t) *t = sw.t;
+    return PDPD_OK;
+}
+
+static void fill_observables(const WorldHost &w, uint32_t n, int64_t t, const double *r, pdpd_observables *o);
+
+int pdpd_snapshot_observables(pdpd_ctx *ctx)
+{
+    int rc = pdpd_snapshot(ctx);
+    if (rc) return rc;
+    if (!ctx->snap_obs_event) HIP_TRY(hipEventCreateWithFlags(&ctx->snap_obs_event, hipEventDisableTiming));
+    const size_t np = std::max<size_t>(1, ctx->n_bblocks) * kObsValues;
+    const size_t no = std::max<size_t>(1, ctx->n_worlds) * kObsValues;
+    if (ctx->snap_obs_partial.n < np || ctx->snap_obs.n < no) {
+        // the stream may still use the old buffers: let their last use finish first
+        HIP_TRY(hipEventSynchronize(ctx->snap_obs_event));
+        if (ctx->snap_obs_partial.n < np) HIP_TRY(dev_alloc(ctx->snap_obs_partial, np));
+        if (ctx->snap_obs.n < no) HIP_TRY(dev_alloc(ctx->snap_obs, no));
+    }
+    if (!ctx->snap_obs_host)
+        HIP_TRY(hipHostMalloc((void **)&ctx->snap_obs_host, (kObsValues + 2) * sizeof(double), hipHostMallocDefault));
+    // the same state the snapshot copied: the buffer the enqueued steps end in
+    if (ctx->n_worlds)
+        HIP_TRY(launch_observables(ctx->state(), ctx->cur, ctx->n_bblocks, ctx->obs_first.p, ctx->n_worlds,
+                                   ctx->snap_obs_partial.p, ctx->snap_obs.p, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->snap_obs_event, ctx->stream));
+    ctx->snap_has_obs = true;
+    return PDPD_OK;
+}
+
+int pdpd_world_snapshot_observables(pdpd_ctx *ctx, int slot, pdpd_observables *out)
+{
+    if (!ctx || !out) return fail(PDPD_ERR_INVALID, "null argument");
+    if (!ctx->snap_has_obs) return fail(PDPD_ERR_STATE, "no snapshot with observables (pdpd_snapshot_observables)");
+    auto it = ctx->snap_worlds.find(slot);
+    if (it == ctx->snap_worlds.end()) return fail(PDPD_ERR_INVALID, "no snapshot of this slot");
+    const pdpd_ctx::SnapWorld sw = it->second;
+    WorldHost *w = find_world(ctx, slot);
+    if (!w) return fail(PDPD_ERR_INVALID, "unknown slot");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->snap_obs_event, 0));
+    double *h = ctx->snap_obs_host;
+    HIP_TRY(hipMemcpyAsync(h, ctx->snap_obs.p + (size_t)sw.index * kObsValues, kObsValues * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->copy_stream));
+    uint32_t *fh = reinterpret_cast<uint32_t *>(h + kObsValues);      // [context-wide, this world]
+    HIP_TRY(hipMemcpyAsync(fh, ctx->snap_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->copy_stream));
+    HIP_TRY(hipMemcpyAsync(fh + 1, ctx->snap_flags.p + 1 + sw.index, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->copy_stream));
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    if (fh[0] & PDPD_FLAG_SCAN_TIMEOUT) return fail(PDPD_ERR_STATE, "cell scan look-back did not complete");
+    if (fh[1]) {
+        w->flags |= fh[1];
+        return fail(PDPD_ERR_STATE, "world " + std::to_string(slot) + ": " + flag_text(fh[1]));
+    }
+    fill_observables(*w, sw.n, sw.t, h, out);     // the snapshot's bead count and time
     return PDPD_OK;
 }
 
@@ -1796,20 +1861,19 @@ static int enqueue_observables(pdpd_ctx *ctx, double *partial, double *out)
     return PDPD_OK;
 }
 
-static void fill_observables(const pdpd_ctx *ctx, const WorldHost &w, const double *r, pdpd_observables *o)
+static void fill_observables(const WorldHost &w, uint32_t n, int64_t t, const double *r, pdpd_observables *o)
 {
     memset(o, 0, sizeof(*o));
     o->slot = w.slot;
-    o->n_beads = w.n;
-    o->t = w.t;
+    o->n_beads = n;
+    o->t = t;
     for (int k = 0; k < 3; k++) o->momentum[k] = r[k];
     o->kinetic = r[3];
     o->virial = r[4];
     o->n_pairs = (uint64_t)r[5];
     const double vol = (double)w.L[0] * (double)w.L[1] * (double)w.L[2];
-    o->temperature = w.n ? r[3] / (3.0 * (double)w.n) : 0.0;
-    o->pressure = ((double)w.n * o->temperature + r[4] / 3.0) / vol;
-    (void)ctx;
+    o->temperature = n ? r[3] / (3.0 * (double)n) : 0.0;
+    o->pressure = ((double)n * o->temperature + r[4] / 3.0) / vol;
 }
 
 int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity, uint32_t *n_out)
@@ -1829,7 +1893,10 @@ int pdpd_get_observables(pdpd_ctx *ctx, pdpd_observables *out, uint32_t capacity
     HIP_TRY(hipMemcpyAsync(r.data(), ctx->obs_out.p, no * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const uint32_t n = std::min(capacity, ctx->n_worlds);
-    for (uint32_t i = 0; i < n; i++) fill_observables(ctx, ctx->worlds.at(ctx->order[i]), &r[(size_t)i * kObsValues], &out[i]);
+    for (uint32_t i = 0; i < n; i++) {
+        const WorldHost &w = ctx->worlds.at(ctx->order[i]);
+        fill_observables(w, w.n, w.t, &r[(size_t)i * kObsValues], &out[i]);
+    }
     return PDPD_OK;
 }
 

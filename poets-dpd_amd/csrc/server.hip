@@ -121,6 +121,8 @@ struct pdpd_server {
     uint64_t next_id = 0;
     pdpd_interval_fn cb = nullptr;
     void *cb_user = nullptr;
+    pdpd_observables_fn ocb = nullptr;     // per-interval observables, no state download
+    void *ocb_user = nullptr;
     pdpd_server_stats stats{};
 
     ~pdpd_server()
@@ -185,6 +187,14 @@ int pdpd_server_set_callback(pdpd_server *s, pdpd_interval_fn fn, void *user)
     return PDPD_OK;
 }
 
+int pdpd_server_set_observables_callback(pdpd_server *s, pdpd_observables_fn fn, void *user)
+{
+    if (!s) return sfail(PDPD_ERR_INVALID, "null server");
+    s->ocb = fn;
+    s->ocb_user = user;
+    return PDPD_OK;
+}
+
 int pdpd_server_submit(pdpd_server *s, const pdpd_world_desc *w, double arrival_s, uint32_t steps, uint64_t *id_out)
 {
     if (!s || !w) return sfail(PDPD_ERR_INVALID, "null argument");
@@ -245,24 +255,38 @@ int pdpd_server_run(pdpd_server *s)
             const bool finished = r->done >= r->steps;
             bool cancel = false;
             uint32_t failed = 0;
-            if (s->cb || finished) {
+            // the request's own world failed: retire it alone; the others do
+            // not depend on it and run on
+            auto world_failed = [&]() -> bool {
+                uint32_t wf = 0;
+                if (pdpd_world_flags(s->ctxs[r->part], r->slot, &wf) == PDPD_OK && wf) failed = wf;
+                return failed != 0;
+            };
+            if (s->ocb) {      // the interval's observables, reduced on the device
+                pdpd_observables o;
+                int rc = pdpd_world_snapshot_observables(s->ctxs[r->part], r->slot, &o);
+                if (rc == PDPD_ERR_STATE) {
+                    if (!world_failed()) return rc;
+                } else if (rc) {
+                    return rc;
+                } else {
+                    cancel = s->ocb(s->ocb_user, r->id, r->done, &o) == 0;
+                }
+            }
+            if ((s->cb || finished) && !failed) {
                 const size_t n = r->res.n_beads;
                 r->x.resize(3 * n); r->v.resize(3 * n); r->f.resize(3 * n);
                 int64_t t = 0;
                 int rc = pdpd_world_snapshot_download(s->ctxs[r->part], r->slot, r->x.data(), r->v.data(),
                                                       r->f.data(), &t);
                 if (rc == PDPD_ERR_STATE) {
-                    // the request's own world failed: retire it alone; the
-                    // others do not depend on it and run on
-                    uint32_t wf = 0;
-                    if (pdpd_world_flags(s->ctxs[r->part], r->slot, &wf) == PDPD_OK && wf) failed = wf;
-                    else return rc;
+                    if (!world_failed()) return rc;
                 } else if (rc) {
                     return rc;
                 }
                 if (s->cb && !failed)
-                    cancel = s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
-                                   r->f.data()) == 0;
+                    cancel |= s->cb(s->cb_user, r->id, r->done, t, (uint32_t)n, r->x.data(), r->v.data(),
+                                    r->f.data()) == 0;
             }
             if (failed) {
                 r->x.clear(); r->v.clear(); r->f.clear();
@@ -354,7 +378,7 @@ int pdpd_server_run(pdpd_server *s)
             // cancelled requests left the context; their interval is discarded
             for (uint32_t p = 0; p < s->ctxs.size(); p++) {
                 if (!s->part_beads[p]) continue;
-                int rc = pdpd_snapshot(s->ctxs[p]);
+                int rc = s->ocb ? pdpd_snapshot_observables(s->ctxs[p]) : pdpd_snapshot(s->ctxs[p]);
                 if (rc) return fail_rc(rc);
             }
             have_pending = !pending.empty();

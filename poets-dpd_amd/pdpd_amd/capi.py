@@ -101,6 +101,7 @@ class Observables(C.Structure):
 
 INTERVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32,
                           C.c_void_p, C.c_void_p, C.c_void_p)
+OBSERVABLES_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(Observables))
 
 # Every symbol include/pdpd_hip.h declares, with its signature.
 # pdpd_pair_record / pdpd_bond_record of pdpd_hip.h
@@ -141,12 +142,15 @@ SIGNATURES = {
     "pdpd_get_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "pdpd_world_observables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Observables)]),
     "pdpd_observables_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "pdpd_snapshot_observables": (C.c_int, [C.c_void_p]),
+    "pdpd_world_snapshot_observables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Observables)]),
     "pdpd_debug_rng": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     "pdpd_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "pdpd_server_destroy": (C.c_int, [C.c_void_p]),
     "pdpd_server_set_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "pdpd_server_set_callback": (C.c_int, [C.c_void_p, INTERVAL_FN, C.c_void_p]),
+    "pdpd_server_set_observables_callback": (C.c_int, [C.c_void_p, OBSERVABLES_FN, C.c_void_p]),
     "pdpd_server_submit": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_double, C.c_uint32,
                                      C.POINTER(C.c_uint64)]),
     "pdpd_server_run": (C.c_int, [C.c_void_p]),
@@ -347,6 +351,16 @@ class Context:
         """Enqueue a copy of every resident world's state behind the work enqueued so far."""
         check(self.lib.pdpd_snapshot(self.h))
 
+    def snapshot_observables(self):
+        """snapshot() plus the FP64 observables of that state, reduced on the device."""
+        check(self.lib.pdpd_snapshot_observables(self.h))
+
+    def snapshot_world_observables(self, slot):
+        """One world's observables from the last snapshot_observables() (waits for them only)."""
+        o = Observables()
+        check(self.lib.pdpd_world_snapshot_observables(self.h, slot, C.byref(o)))
+        return o.as_dict()
+
     def snapshot_download(self, slot):
         """The last snapshot of the slot as (x, v, f, t), without waiting for later steps."""
         n = self.worlds[slot][0]
@@ -526,6 +540,7 @@ class Server:
         self.h = h
         self.sizes = {}
         self._cb = None
+        self._ocb = None
 
     def close(self):
         if self.h:
@@ -556,6 +571,19 @@ class Server:
             return 1 if fn(int(rid), int(done), int(t), arr(x), arr(v), arr(f)) else 0
         self._cb = INTERVAL_FN(tramp)
         check(self.lib.pdpd_server_set_callback(self.h, self._cb, None))
+
+    def set_observables_callback(self, fn):
+        """fn(request_id, steps_done, observables_dict) -> bool (False cancels),
+        after every interval, from sums reduced on the device (no state copy)."""
+        if fn is None:
+            self._ocb = None
+            check(self.lib.pdpd_server_set_observables_callback(self.h, OBSERVABLES_FN(), None))
+            return
+
+        def tramp(user, rid, done, obs):
+            return 1 if fn(int(rid), int(done), obs.contents.as_dict()) else 0
+        self._ocb = OBSERVABLES_FN(tramp)
+        check(self.lib.pdpd_server_set_observables_callback(self.h, self._ocb, None))
 
     def submit(self, world, arrival_s, steps):
         d, keep = world_desc(world)

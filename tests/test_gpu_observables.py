@@ -92,3 +92,33 @@ def test_water16_long_run_statistics_vs_reference(gpu_lib):
     m = r > 0.3
     assert np.abs(gg[m] - gr[m]).max() < 0.05
     assert abs(gg[r > 1.5].mean() - 1.0) < 0.03
+
+
+def test_snapshot_observables_equal_synchronous_ones(gpu_lib):
+    """pdpd_snapshot_observables reduces the snapshot's state behind the
+    enqueued steps; read after more steps were enqueued, the sums equal those
+    pdpd_world_observables takes of the same state in another context, bit for
+    bit (same kernels, same per-world block partition)."""
+    rng = np.random.default_rng(21)
+    worlds = [W.make_world((7, 6, 8), bonded_fraction=0.2, seed=31, a_matrix=W.random_a_matrix(rng)),
+              W.make_world((5, 9, 6), seed=32)]
+    _say("snapshot observables of 2 worlds after 12 steps, 5 more enqueued behind")
+    ctx = capi.Context(0)
+    slots = [ctx.add(w) for w in worlds]
+    ctx.run(12)
+    ctx.snapshot_observables()
+    ctx.run(5)
+    got = [ctx.snapshot_world_observables(s) for s in slots]
+    ctx.sync()
+    ctx.close()
+    ref = capi.Context(0)
+    rs = [ref.add(w) for w in worlds]
+    ref.run(12)
+    ref.sync()
+    want = [ref.observables(s) for s in rs]
+    ref.close()
+    for g, w_ in zip(got, want):
+        assert g["t"] == w_["t"] == 12
+        for k in ("kinetic", "virial", "n_pairs", "temperature", "pressure", "n_beads"):
+            assert g[k] == w_[k], k
+        assert g["momentum"] == w_["momentum"]

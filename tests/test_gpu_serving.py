@@ -13,6 +13,10 @@ from pdpd_amd import world as W
 pytestmark = pytest.mark.gpu
 
 
+def _say(msg):
+    print(msg, flush=True)
+
+
 def _requests(n, seed=3):
     rng = np.random.default_rng(seed)
     reqs = []
@@ -135,3 +139,42 @@ def test_world_flags_name_the_failing_world(gpu_lib):
     x, v, f, t = ctx.download(s_ok)
     assert t == 5
     ctx.close()
+
+
+def test_observables_callback_matches_solo_runs(gpu_lib):
+    """The serving loop's observables callback: after every interval each
+    request's FP64 sums, reduced on the device from the interval's snapshot
+    (no copy of the state), equal the sums of the same world run alone to the
+    same step, bit for bit; results still equal solo runs."""
+    rng = np.random.default_rng(23)
+    reqs = [(W.make_world((6, 5, 7), bonded_fraction=0.2, seed=40 + i, a_matrix=W.random_a_matrix(rng)), st)
+            for i, st in enumerate((30, 20, 40))]
+    _say("observables callback: 3 requests, intervals of 10")
+    srv = capi.Server(device=0, interval_steps=10, time_scale=0.0)
+    seen = {}
+
+    def on_obs(rid, done, obs):
+        seen.setdefault(rid, []).append((done, obs))
+        return True
+    srv.set_observables_callback(on_obs)
+    ids = [srv.submit(w, 0.0, st) for w, st in reqs]
+    srv.run()
+    results = {rid: srv.result(rid) for rid in ids}
+    srv.close()
+    for rid, (w, st) in zip(ids, reqs):
+        assert results[rid]["status"] == 0
+        got = seen[rid]
+        assert [d for d, _ in got] == list(range(10, st + 1, 10))
+        c = capi.Context(0)
+        s = c.add(w)
+        for done, obs in got:
+            c.run(10)
+            c.sync()
+            want = c.observables(s)
+            assert obs["t"] == want["t"] == done
+            for k in ("kinetic", "virial", "n_pairs", "temperature", "pressure"):
+                assert obs[k] == want[k], (rid, done, k)
+            assert obs["momentum"] == want["momentum"]
+        x, v, f, t = c.download(s)
+        c.close()
+        assert np.array_equal(results[rid]["x"], x) and np.array_equal(results[rid]["f"], f)

@@ -207,6 +207,22 @@ for rid, w in ((ids[0], good[0]), (ids[2], good[1])):
     ok &= r["status"] == 0 and np.array_equal(r["x"], x) and np.array_equal(r["v"], v) and np.array_equal(r["f"], f)
 print("failed_request", "ok" if ok else "WRONG", st, rb, flush=True)
 srv.close()
+# observables per interval from the device (no state copy) equal a solo run's
+w = W.make_world((4, 5, 4), bonded_fraction=0.2, seed=703)
+srv = capi.Server(device=0, interval_steps=10, time_scale=0.0)
+obs = []
+srv.set_observables_callback(lambda rid, done, o: obs.append((done, o)) or True)
+rid = srv.submit(w, 0.0, 20)
+srv.run()
+srv.close()
+c = capi.Context(0); s = c.add(w)
+ok = [d for d, _ in obs] == [10, 20]
+for done, o in obs:
+    c.run(10)
+    want = c.observables(s)
+    ok &= all(o[k] == want[k] for k in ("t", "kinetic", "virial", "n_pairs", "momentum"))
+c.close()
+print("observables_callback", "ok" if ok else "WRONG", flush=True)
 """
 
 
@@ -222,3 +238,4 @@ def test_emulated_serving_pipeline(emu_build):
     assert r.returncode == 0, r.stderr[-3000:]
     assert "serving ok" in r.stdout and "ensemble ok" in r.stdout, r.stdout
     assert "failed_request ok" in r.stdout, r.stdout
+    assert "observables_callback ok" in r.stdout, r.stdout
