@@ -29,6 +29,9 @@ Extra keys:
                 500 steps (each request's FP64 observables, reduced on the
                 device), an HBM budget that holds only part of the requests;
                 p50/p99 latency, admission wait and aggregate bead-steps/s
+  served_sustained  the same kind of sweep, 64 requests at 3/s: an offered
+                load (about 2.4e9 bead-steps/s) below the device's capacity, so
+                latency is service time rather than queueing
   cpu_baseline  the reference's NaiveDPDEngineHalfMerge compiled from its
                 sources (-O3 -ffast-math), one thread, bounded sample of the
                 polymer world
@@ -224,10 +227,10 @@ def active_parts(ens):
     return sum(1 for c in ens.parts if c.worlds)
 
 
-def run_served(args, capi, W):
+def run_served(args, capi, W, n_requests, rate):
     """Served sweep with continuous batching: arrivals join the resident
     requests at interval boundaries while the HBM budget allows."""
-    reqs = W.served_requests(args.serve, seed=1)
+    reqs = W.served_requests(n_requests, seed=1, rate=rate)
     srv = capi.Server(device=0, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), interval_steps=500,
                       time_scale=1.0, contexts=args.serve_contexts)
     seen = [0]
@@ -265,7 +268,9 @@ def run_served(args, capi, W):
             "relayout_max_ms": 1e3 * st["relayout_max_s"], "callbacks": seen[0],
             "callback": "observables (FP64 momentum, temperature, pressure per interval, reduced on the device)",
             "mean_interval_temperature": float(sum(temps) / len(temps)) if temps else None, "interval_steps": 500,
-            "arrival_rate_per_s": 50.0, "last_arrival_s": reqs[-1]["arrival"] if reqs else 0.0,
+            "arrival_rate_per_s": rate,
+            "offered_bead_steps_per_s": sum(3 * r["L"][0] ** 3 * r["steps"] for r in reqs) / reqs[-1]["arrival"]
+            if reqs else 0.0, "last_arrival_s": reqs[-1]["arrival"] if reqs else 0.0,
             "hbm_budget_gib": args.serve_budget_gib, "contexts": args.serve_contexts}
 
 
@@ -305,6 +310,9 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the plain-engine large world")
     ap.add_argument("--serve", type=int, default=256, help="served sweep with this many requests (0 = skip)")
     ap.add_argument("--serve-budget-gib", type=float, default=0.5)
+    ap.add_argument("--serve-sustained", type=int, default=64,
+                    help="a second served sweep at an arrival rate the device keeps up with (0 = skip)")
+    ap.add_argument("--serve-sustained-rate", type=float, default=3.0)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
     ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "polymer32", "large128", "served"),
                     default=None, help="run one workload only (profiling runs); ensemble-profile runs only the "
@@ -345,7 +353,7 @@ def main():
         return
 
     if args.only == "served":
-        print(json.dumps(run_served(args, capi, W)), flush=True)
+        print(json.dumps(run_served(args, capi, W, args.serve, 50.0)), flush=True)
         return
 
     # ---- headline: the 256-world ensemble, batched engine
@@ -426,7 +434,10 @@ def main():
                                               "pair_force_large128_pmc.md")
     if args.serve:
         log(f"bench: served sweep of {args.serve} requests")
-        out["served"] = run_served(args, capi, W)
+        out["served"] = run_served(args, capi, W, args.serve, 50.0)
+    if args.serve_sustained:
+        log(f"bench: served sweep at {args.serve_sustained_rate}/s, {args.serve_sustained} requests")
+        out["served_sustained"] = run_served(args, capi, W, args.serve_sustained, args.serve_sustained_rate)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         out.update(cpu_baseline(W.config_polymer(seed=1), args.cpu_seconds))
