@@ -13,9 +13,10 @@ timed steps replay a graph that already exists.
 Extra keys:
   roofline      the pair-force kernel of the headline workload: algorithmic
                 bytes per launch / its average launch duration, HIP events on
-                the engine's stream around each kernel of chained steps (as the
-                step runs them); traffic = HBM bytes per launch from the
-                committed rocprofv3 --pmc summary of the same workload
+                each context's stream around each kernel in a profiled pass
+                (each context alone, right after tuning, as --only
+                ensemble-profile runs it); traffic = HBM bytes per launch from
+                the committed rocprofv3 --pmc summary of that pass
   batched       the same ensemble for 5000 steps, against the same 256 worlds
                 run one at a time through the plain engine for 5000 steps each;
                 one_context: the ensemble in a single context (one launch set
@@ -381,12 +382,14 @@ def main():
         log(f"bench: ensemble: {total * args.steps / el:.4e} bead-steps/s")
         ens.close()
         return
+    # the profiled pass first, from the freshly tuned state: exactly what
+    # --only ensemble-profile runs, whose rocprofv3 trace is committed
+    prof = kernel_profile(ens.parts, args.profile_steps)
     w_done = warm(ens, args.warmup)
     el = timed(ens, args.steps)
     value = total * args.steps / el
     step_ms = el * 1e3 / args.steps
     log(f"bench: {value:.4e} bead-steps/s ({step_ms:.4f} ms/step)")
-    prof = kernel_profile(ens.parts, args.profile_steps)
     roofline = roofline_of(total, active_parts(ens), prof, step_ms, traffic_from_profiles("pair_force_ensemble256_pmc.md"))
 
     out = {"metric": METRIC, "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
