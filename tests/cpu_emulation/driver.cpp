@@ -143,12 +143,12 @@ int main(int argc, char **argv)
         {"wall_v6", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=6,force_grid=4"},
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
-        {"dense_brick3", {5, 4, 6}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=3"},
+        {"dense_brick3", {4, 4, 3}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=3"},    // every brick beyond LDS
         {"poly_brick1", {6, 7, 8}, 0.2, false, 0.0, 2, "force_kernel=1,force_brick=1"},
         {"wall_brick2", {6, 5, 7}, 0.2, true, 1.0, 2, "force_kernel=1,force_brick=2"},
         {"poly_brick5", {9, 8, 10}, 0.1, false, 0.0, 2, "force_kernel=1,force_brick=5"},
         {"poly_brick_grid", {9, 8, 10}, 0.1, false, 0.0, 3, "force_kernel=1,force_grid=3"},
-        {"dense_slab6", {5, 4, 6}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=6"},
+        {"dense_slab6", {4, 4, 3}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=6"},     // every slab beyond LDS
         {"poly_slab7", {6, 7, 8}, 0.2, false, 0.0, 2, "force_kernel=1,force_brick=7"},
         {"wall_slab8", {6, 5, 7}, 0.2, true, 1.0, 2, "force_kernel=1,force_brick=8"},
         {"poly_slab9", {19, 6, 5}, 0.1, false, 0.0, 3, "force_kernel=1,force_brick=9"},
