@@ -315,7 +315,7 @@ def main():
                     help="a second served sweep at an arrival rate the device keeps up with (0 = skip)")
     ap.add_argument("--serve-sustained-rate", type=float, default=3.0)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
-    ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "polymer32", "large128", "served"),
+    ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "water16", "polymer32", "large128", "served"),
                     default=None, help="run one workload only (profiling runs); ensemble-profile runs only the "
                                        "headline's profiled pass (no warm-up, no timed steps)")
     ap.add_argument("--graph-steps", type=int, default=None,
@@ -339,8 +339,8 @@ def main():
     from pdpd_amd import world as W
     capi.load()
 
-    if args.only in ("polymer32", "large128"):
-        world = W.config_polymer(seed=1) if args.only == "polymer32" else W.config_large(seed=1)
+    if args.only in ("water16", "polymer32", "large128"):
+        world = {"water16": W.config_water, "polymer32": W.config_polymer, "large128": W.config_large}[args.only](seed=1)
         ctx = capi.Context(0)
         ctx.add(world)
         if args.graph_steps is not None:
