@@ -66,3 +66,17 @@ def test_context_without_device_fails_loudly():
              % os.path.join(os.path.dirname(HEADER), "..", "poets-dpd_amd"))
     out = subprocess.run([sys.executable, "-c", probe], capture_output=True, text=True, timeout=120).stdout
     assert out.startswith("OK") or out.startswith("ERR -3") or out.startswith("ERR -1")
+
+
+def test_new_entry_points_reject_null_arguments_without_a_device():
+    """The group autotuner, snapshot observables and the server's observables
+    callback check their arguments before any device call."""
+    import ctypes as C
+    lib = capi.load()
+    buf = C.create_string_buffer(64)
+    assert lib.pdpd_autotune_group(None, 0, 4, buf, 64) == -1
+    arr = (C.c_void_p * 2)(None, None)
+    assert lib.pdpd_autotune_group(arr, 2, 4, buf, 64) == -1
+    assert lib.pdpd_snapshot_observables(None) == -1
+    assert lib.pdpd_world_snapshot_observables(None, 0, None) == -1
+    assert lib.pdpd_server_set_observables_callback(None, capi.OBSERVABLES_FN(), None) == -1
