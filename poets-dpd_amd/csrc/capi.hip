@@ -65,11 +65,14 @@ constexpr uint32_t kMaxBox = 1024;
 // (1.77 against 1.74 ms): on by default up to this many resident beads.
 constexpr uint32_t kFuseAutoBeads = 1u << 20;
 // force_variant=auto: variant 4 (smaller pair queue, fewer registers, 6 waves
-// per SIMD) from this many resident beads, variant 0 below (measured: 128^3
-// world 1.19 against 1.34 ms per pair-force launch, 256-world ensemble over 12
-// contexts of 0.5M beads +8 %; 32^3 and 16^3 worlds 1 us slower)
+// per SIMD) from kVariantLargeBeads resident beads (measured: 128^3 world 1.19
+// against 1.34 ms per pair-force launch, 256-world ensemble over 12 contexts of
+// 0.5M beads +8 %; 32^3 and 16^3 worlds 1 us slower), variant 7 (a wave per
+// stencil row) below kVariantSmallBeads (16^3 world 17.8 against 19.3 us; at
+// 32^3 it takes 2.6 times variant 0's time), variant 0 between
 constexpr uint32_t kVariantAuto = UINT32_MAX;
 constexpr uint32_t kVariantLargeBeads = 1u << 18;
+constexpr uint32_t kVariantSmallBeads = 1u << 14;
 constexpr uint64_t kBandMinCells = 64 * 64;   // y-z plane size from which force blocks are listed in bands
 constexpr uint32_t kBandRows = 16;            // default band height (y rows)
 constexpr uint32_t kMaxPolymerLength = 127;
@@ -250,7 +253,8 @@ struct pdpd_ctx {
     bool fuse() const { return fuse_mode == 1 || (fuse_mode == 2 && n_beads <= kFuseAutoBeads); }
     uint32_t variant() const
     {
-        return force_variant != kVariantAuto ? force_variant : (n_beads >= kVariantLargeBeads ? 4u : 0u);
+        if (force_variant != kVariantAuto) return force_variant;
+        return n_beads >= kVariantLargeBeads ? 4u : (n_beads < kVariantSmallBeads ? 7u : 0u);
     }
 
     // graph cache
@@ -1689,8 +1693,9 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
          "\tpair-force kernel variant: 0 = 3 candidate loads in flight, bonded sums after the middle plane's pairs;"
          " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave;"
          " 4 = 2 loads, middle plane, 128-pair queues, 6 waves per SIMD; 5 = 3 loads, 192-pair queues, 6 waves;"
-         " 6 = 2 loads, fourth wave, 128-pair queues, 6 waves; auto = 4 from " + std::to_string(kVariantLargeBeads) +
-         " resident beads, else 0\n";
+         " 6 = 2 loads, fourth wave, 128-pair queues, 6 waves; 7 = one wave per stencil row (nine per workgroup),"
+         " 3 loads, 192-pair queues; auto = 4 from " + std::to_string(kVariantLargeBeads) +
+         " resident beads, 7 below " + std::to_string(kVariantSmallBeads) + ", else 0\n";
     s += "fused_integrate=" + std::to_string(ctx->fuse_mode) +
          "\tchained steps, the pair-force kernel integrating the next step: 0 = off, 1 = on, 2 = on up to " +
          std::to_string(kFuseAutoBeads) + " resident beads\n";

@@ -32,7 +32,7 @@ struct StepLaunch {
     const BrickRec *bricks;
 };
 
-constexpr uint32_t kForceVariants = 7;
+constexpr uint32_t kForceVariants = 8;
 
 // Events recorded between the kernels of one step (profiling runs only):
 // [0] integrate [1] scan [2] scatter [3] reorder [4] force (+ bonded terms) [5] stationary [6]
@@ -54,6 +54,9 @@ void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz);
 // (force_slab.hip); same inputs, outputs and bits. The brick shapes from
 // kSlabShape0 on name its shapes.
 hipError_t launch_force_slab(const StepLaunch &L, uint32_t shape, hipStream_t st);
+// Pair forces with one wave per stencil row, nine per workgroup (force_rows.hip):
+// three candidate loads per lane in flight, 192-pair queues.
+hipError_t launch_force_rows(const StepLaunch &L, hipStream_t st);
 uint32_t slab_shape_count();
 void slab_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);

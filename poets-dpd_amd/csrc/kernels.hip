@@ -1045,6 +1045,9 @@ hipError_t launch_force(const StepLaunch &L, hipStream_t st)
         case 4: k_force<2, 6, false, false, 128><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         case 5: k_force<3, 6, false, false, 192><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         case 6: k_force<2, 6, true, false, 128><<<fg, kForceThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
+        // one wave per stencil row (nine per workgroup): shorter dependent chains
+        // per wave, for worlds too small to give each SIMD several waves
+        case 7: return launch_force_rows(L, st);
         default: k_force<3, 5, false, false><<<fg, t3, 0, st>>>(Sp, nxt, L.n_force_blocks, ns); break;
         }
     }

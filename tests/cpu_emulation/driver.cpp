@@ -141,6 +141,9 @@ int main(int argc, char **argv)
         {"poly_v4", {6, 7, 8}, 0.1, false, 0.0, 2, "force_variant=4"},
         {"wall_v5", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=5"},
         {"wall_v6", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=6,force_grid=4"},
+        {"poly_v7", {6, 7, 8}, 0.2, false, 0.0, 3, "force_variant=7"},             // a wave per stencil row
+        {"wall_v7", {6, 5, 7}, 0.2, true, 1.0, 2, "force_variant=7,force_grid=4"},
+        {"dense_v7", {3, 4, 3}, 0.0, false, 0.0, 2, "force_variant=7"},           // rows in several rounds
         {"poly_scan3", {9, 8, 7}, 0.1, false, 0.0, 3, "scan_lookback_tiles=0"},
         {"dense", {3, 4, 3}, 0.0, false, 0.0, 2, nullptr},
         {"dense_brick3", {4, 4, 3}, 0.0, false, 0.0, 2, "force_kernel=1,force_brick=3"},    // every brick beyond LDS

@@ -235,6 +235,8 @@ def test_autotune_leaves_state_unchanged(gpu_lib):
                                  dict(force_variant=1), dict(force_variant=2), dict(force_variant=3),
                                  dict(force_variant=4), dict(force_variant=5), dict(force_variant=6),
                                  dict(force_variant=4, force_grid=64, graph_steps=6),
+                                 dict(force_variant=7), dict(force_variant=7, force_grid=64, graph_steps=6),
+                                 dict(force_variant=7, fused_integrate=0),
                                  dict(fused_integrate=0), dict(fused_integrate=1, graph_steps=6),
                                  dict(fused_integrate=1, graph_steps=0, force_grid=64),
                                  dict(force_kernel=1), dict(force_kernel=1, force_brick=1),
@@ -303,7 +305,8 @@ def test_dense_cell_matches_oracle(gpu_lib):
     """1500 beads in one cell (the reference AVX2 engine would throw on more
     than 32, dpd_engine_avx2_half_merge_tbb.hpp:816-820): the pair lists flush
     as often as needed, the brick kernel stages the rows in chunks, and the
-    step still matches the oracle bit for bit."""
+    step still matches the oracle bit for bit; the row-wave kernel finishes
+    the dense rows over many rounds."""
     b = W.WorldBuilder((4, 4, 4), dt=0.001, seed=1)
     b.add_bead_type("A")
     b.set_interaction("A", "A", 25.0, 4.5)
@@ -313,9 +316,10 @@ def test_dense_cell_matches_oracle(gpu_lib):
         b.add_polymer(m, 1.0 + 0.9 * r.random(3))
     w = b.build()
     ref = O.F32State(w).step(1)
-    for kernel in (0, 1):       # per-slot kernel; brick kernel (its path for bricks beyond the LDS capacity)
-        _say(f"dense cell: 1500 beads in one cell, force_kernel={kernel}")
-        ctx, slot = run_gpu(w, 1, force_kernel=kernel)
+    # per-slot kernel; brick kernel (its path for bricks beyond the LDS capacity); row waves
+    for cfg in (dict(force_kernel=0), dict(force_kernel=1), dict(force_kernel=0, force_variant=7)):
+        _say(f"dense cell: 1500 beads in one cell, {cfg}")
+        ctx, slot = run_gpu(w, 1, **cfg)
         x, ww, f, t = ctx.download_f32(slot)
         ctx.close()
         assert np.array_equal(x.view(np.uint32), ref.x.view(np.uint32))

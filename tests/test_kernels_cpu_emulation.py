@@ -77,6 +77,22 @@ for (name, w), var in zip(worlds.items(), (4, 1, 5, 3, 6, 2)):
             np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
     print(name, "bitwise" if same else "DIFFERENT", flush=True)
     ctx.close()
+# the row-wave variant (one wave per stencil row; the dense world takes
+# several rounds per row)
+for var in (7,):
+    same = True
+    for name in ("poly_6x7x8", "wall_bent6", "dense3", "lattice4"):
+        ctx = capi.Context(0)
+        ctx.set_config("force_variant", var)
+        slot = ctx.add(worlds[name])
+        ctx.run(2)
+        x, ww, f, t = ctx.download_f32(slot)
+        ref = O.F32State(worlds[name]).step(2)
+        same &= (np.array_equal(x.view(np.uint32), ref.x.view(np.uint32)) and
+                 np.array_equal(ww.view(np.uint32), ref.w.view(np.uint32)) and
+                 np.array_equal(f.view(np.uint32), ref.f.view(np.uint32)))
+        ctx.close()
+    print(f"variant{var}", "bitwise" if same else "DIFFERENT", flush=True)
 # every brick shape of the LDS brick kernel (the dense world takes the path for
 # bricks beyond the LDS capacity), and the per-slot kernel
 for shape in range(int(capi.Context(0).get_config("force_brick_shapes"))):
@@ -142,8 +158,10 @@ def test_emulated_step_equals_oracle_bitwise(emu_build):
     assert r.returncode == 0, r.stderr[-3000:]
     lines = r.stdout.split("\n")
     shapes = [l for l in lines if l.startswith("brick")]
+    rows = [l for l in lines if l.startswith("variant")]
     assert len(shapes) == 10 and "DIFFERENT" not in r.stdout, r.stdout     # 6 bricks, 4 slabs
-    assert sum(l.endswith("bitwise") for l in lines) == 10 + len(shapes), r.stdout
+    assert len(rows) == 1, r.stdout
+    assert sum(l.endswith("bitwise") for l in lines) == 10 + len(shapes) + len(rows), r.stdout
 
 
 SERVE_PROBE = r"""
