@@ -68,8 +68,8 @@ constexpr uint32_t kFuseAutoBeads = 1u << 20;
 // per SIMD) from kVariantLargeBeads resident beads (measured: 128^3 world 1.19
 // against 1.34 ms per pair-force launch, 256-world ensemble over 12 contexts of
 // 0.5M beads +8 %; 32^3 and 16^3 worlds 1 us slower), variant 7 (a wave per
-// stencil row) below kVariantSmallBeads (16^3 world 17.8 against 19.3 us; at
-// 32^3 it takes 2.6 times variant 0's time), variant 0 between
+// stencil row) below kVariantSmallBeads (16^3 world 14.3 against 19.3 us; at
+// 20^3 23.5 against 19.9, at 32^3 2.5 times variant 0's time), variant 0 between
 constexpr uint32_t kVariantAuto = UINT32_MAX;
 constexpr uint32_t kVariantLargeBeads = 1u << 18;
 constexpr uint32_t kVariantSmallBeads = 1u << 14;
@@ -1694,7 +1694,7 @@ int pdpd_list_config(pdpd_ctx *ctx, char *buf, size_t buflen)
          " 1 = 4 loads, bonded sums by a fourth wave; 2 = 2 loads, middle plane; 3 = 3 loads, fourth wave;"
          " 4 = 2 loads, middle plane, 128-pair queues, 6 waves per SIMD; 5 = 3 loads, 192-pair queues, 6 waves;"
          " 6 = 2 loads, fourth wave, 128-pair queues, 6 waves; 7 = one wave per stencil row (nine per workgroup),"
-         " 3 loads, 192-pair queues; auto = 4 from " + std::to_string(kVariantLargeBeads) +
+         " 8 loads, 512-pair queues; auto = 4 from " + std::to_string(kVariantLargeBeads) +
          " resident beads, 7 below " + std::to_string(kVariantSmallBeads) + ", else 0\n";
     s += "fused_integrate=" + std::to_string(ctx->fuse_mode) +
          "\tchained steps, the pair-force kernel integrating the next step: 0 = off, 1 = on, 2 = on up to " +

@@ -62,7 +62,8 @@ template <int kW, int kRQ>
 __global__ __launch_bounds__(kRowThreads) void k_force_rows(DeviceState S, int nxt, uint32_t n_blocks,
                                                             const NextBuffers *next)
 {
-    static_assert(kRQ >= kForceBeads * kW && kRQ <= 256, "a chunk takes at least one batch; positions are 8-bit");
+    static_assert(kRQ >= kForceBeads * kW && kRQ <= 65536, "a chunk takes at least one batch");
+    using Pos = typename std::conditional<(kRQ > 256), uint16_t, uint8_t>::type;   // queue positions
     static_assert(kRowList >= kW, "a lane lists up to kW pairs per batch");
     // per row wave: queued pairs (candidate slot | image codes in bits 26..31,
     // home lane), their evaluated (unit vector, force magnitude), and per home
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kRowThreads) void k_force_rows(DeviceState S, int n
     __shared__ uint32_t s_qslot[kRowWaves][kRQ];
     __shared__ uint8_t s_qhome[kRowWaves][kRQ];
     __shared__ float4 s_qres[kRowWaves][kRQ];
-    __shared__ uint8_t s_list[kRowList][kRowThreads];
+    __shared__ Pos s_list[kRowList][kRowThreads];
     __shared__ uint8_t s_nlist[kRowWaves][kForceBeads];
     // per row wave: a chunk waits to be added; the row is scanned to its end
     __shared__ uint32_t s_pending[kRowWaves], s_complete[kRowWaves];
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(kRowThreads) void k_force_rows(DeviceState S, int n
                             const uint32_t qp = at + (uint32_t)__popcll(bal[j] & below);
                             qslot[qp] = slot[j] | cxy | (k + j < na ? zcA : zcB);
                             qhome[qp] = (uint8_t)lane;
-                            s_list[listed++][tid] = (uint8_t)qp;
+                            s_list[listed++][tid] = (Pos)qp;
                         }
                         at += (uint32_t)__popcll(bal[j]);
                     }
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kRowThreads) void k_force_rows(DeviceState S, int n
                     const int rw = w + next_row;
                     if (!s_pending[rw]) break;
                     const uint32_t n = s_nlist[rw][lane];
-                    const uint8_t *lst = &s_list[0][rw * kForceBeads + lane];
+                    const Pos *lst = &s_list[0][rw * kForceBeads + lane];
                     for (uint32_t c = 0; c < n; c++) {
                         const float4 q = s_qres[rw][lst[c * kRowThreads]];
                         px = fmaf(q.x, q.w, px);
@@ -328,7 +329,7 @@ hipError_t launch_force_rows(const StepLaunch &L, hipStream_t st)
     const NextBuffers *ns = L.fuse ? L.next + (L.cur * 2 + L.cpar) : nullptr;
     const uint32_t fg = L.force_grid ? std::min<uint32_t>(L.force_grid, L.n_force_blocks) : L.n_force_blocks;
     if (!fg) return hipSuccess;
-    k_force_rows<3, 192><<<fg, kRowThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns);
+    k_force_rows<8, 512><<<fg, kRowThreads, 0, st>>>(Sp, nxt, L.n_force_blocks, ns);
     return hipGetLastError();
 }
 

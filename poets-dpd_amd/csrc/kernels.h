@@ -55,7 +55,7 @@ void brick_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by, uint32_t *bz);
 // kSlabShape0 on name its shapes.
 hipError_t launch_force_slab(const StepLaunch &L, uint32_t shape, hipStream_t st);
 // Pair forces with one wave per stencil row, nine per workgroup (force_rows.hip):
-// three candidate loads per lane in flight, 192-pair queues.
+// eight candidate loads per lane in flight, 512-pair queues.
 hipError_t launch_force_rows(const StepLaunch &L, hipStream_t st);
 uint32_t slab_shape_count();
 void slab_shape_dims(uint32_t shape, uint32_t *bx, uint32_t *by);
