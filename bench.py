@@ -186,6 +186,30 @@ def roofline_of(n_beads, launches, prof, step_ms, traffic):
             "kernel_ms_per_step": {k[2:-3]: prof[k] for k in KERNELS}}
 
 
+def run_sizes(capi, W, sides=(8, 12, 16, 20, 24, 32, 48, 64, 96, 128), seconds=0.5):
+    """The plain engine on water worlds of side L (density 3, L^3 cells) at its
+    default launch shape: per side the step time over a run of about
+    `seconds` after a warm-up (force_variant=auto picks the pair-force kernel
+    by the number of resident beads)."""
+    rows = []
+    for L in sides:
+        world = W.make_world((L, L, L), seed=L)
+        ctx = capi.Context(0)
+        ctx.add(world)
+        warm(ctx, 40)
+        probe = max(20, int(2e4 / max(1.0, world.n_beads / 1e4)))
+        el = timed(ctx, probe)
+        steps = max(20, int(seconds / (el / probe)))
+        el = timed(ctx, steps)
+        rows.append({"side": L, "beads": world.n_beads, "steps": steps, "us_per_step": el * 1e6 / steps,
+                     "bead_steps_per_s": world.n_beads * steps / el})
+        log(f"bench: sizes: {L}^3 ({world.n_beads} beads) {el * 1e6 / steps:.1f} us/step")
+        ctx.close()
+    return {"metric": "plain-engine step time by world size", "unit": "us/step", "data": "synthetic",
+            "config": {"workload": "water worlds L^3 cells, density 3, a=25, gamma=4.5, dt=0.01, default launch shape"},
+            "sizes": rows}
+
+
 def make_ensemble(capi, worlds, parts):
     total = sum(w.n_beads for w in worlds)
     parts = parts if parts else capi.suggested_parts(total)
@@ -315,7 +339,8 @@ def main():
                     help="a second served sweep at an arrival rate the device keeps up with (0 = skip)")
     ap.add_argument("--serve-sustained-rate", type=float, default=3.0)
     ap.add_argument("--serve-contexts", type=int, default=1, help="contexts the served requests are spread over")
-    ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "water16", "polymer32", "large128", "served"),
+    ap.add_argument("--only", choices=("ensemble", "ensemble-profile", "water16", "polymer32", "large128", "served",
+                                       "sizes"),
                     default=None, help="run one workload only (profiling runs); ensemble-profile runs only the "
                                        "headline's profiled pass (no warm-up, no timed steps)")
     ap.add_argument("--graph-steps", type=int, default=None,
@@ -351,6 +376,10 @@ def main():
         el = timed(ctx, args.steps)
         log(f"bench: {args.only}: {world.n_beads * args.steps / el:.4e} bead-steps/s")
         ctx.close()
+        return
+
+    if args.only == "sizes":
+        print(json.dumps(run_sizes(capi, W)), flush=True)
         return
 
     if args.only == "served":
